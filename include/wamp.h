@@ -1,0 +1,126 @@
+/*
+ * wamp.h -- C-ABI of libwamp.so, the MI355X-native (gfx950) five-equation
+ * stiffened-gas solver step.
+ *
+ * The reference (aschamarthi/Wave-Multiphase) has no plugin API: its
+ * boundary is the set of wp.launch call sites inside main()'s closures
+ * (sainath_SC_it_works.py, "SC", lines 1894-1995; Sainath_M10.py, "M10",
+ * lines 1815-1907).  Each entry point below replaces one closure (or one
+ * group of launches) and cites it.
+ *
+ * Conventions
+ *  - State arrays are SoA FP64, one array per conserved field in the order
+ *    [a1*rho1, a2*rho2, rho*u, rho*v, E, a1] (SC:777, 1842-1844), each of
+ *    shape (nx+2g, ld) row-major with j (y) contiguous, interior cells at
+ *    [g, g+nx) x [g, g+ny) (wp.array2d layout, SC:1839-1840).  ld >= ny+2g.
+ *  - Pointers are device pointers (e.g. torch.Tensor.data_ptr()).  The
+ *    caller owns every state, flux and primitive buffer; the context owns
+ *    only scratch it allocates in wamp_ctx_create.
+ *  - Every call enqueues on `stream` (a hipStream_t; NULL = default stream)
+ *    and returns without synchronising.
+ *  - Return value: 0 ok, > 0 a hipError_t, < 0 an argument error.  The
+ *    message of the last failure on this thread is wamp_last_error().
+ */
+#ifndef WAMP_H
+#define WAMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wamp_ctx wamp_ctx;
+
+/* Grid: SC:26-32.  bc: x-lo, x-hi, y-lo, y-hi; 0 = zero-gradient copy of the
+   edge cell (bc_x_kernel SC:1616-1646), 1 = slip wall, the same copy with
+   the wall-normal momentum negated (bc_y_kernel SC:1650-1682). */
+typedef struct wamp_grid {
+    int32_t nx, ny, g, ld;
+    double dx, dy;
+    int32_t bc[4];
+} wamp_grid;
+
+/* Two stiffened-gas phases (gamma, p_inf), SC:40-43. */
+typedef struct wamp_eos {
+    double gamma1, pinf1, gamma2, pinf2;
+} wamp_eos;
+
+/* Scheme switches.  base: 0 MUSCL (SC:1061-1073), 1 MP5 (SC:353-405).
+   shear: 0 none (M10), 1 central4 (SC:1074-1082), 2 central6 (SC:407-412),
+   Ducros-gated (SC:866-916).  vf_thinc: 0 gated (SC:1085-1092), 1 always
+   (M10:971-979).  face_pp: 0 first-order fallback (SC:1102-1115), 1 Wong
+   face limiter + pressure floor (M10:1007-1025).  repair: enforce_admissible
+   after every stage (USE_EMERGENCY_REPAIR, SC:37).  cfl: SC:34. */
+typedef struct wamp_scheme {
+    int32_t base, shear, vf_thinc, face_pp, repair;
+    double cfl;
+} wamp_scheme;
+
+#define WAMP_PREC_FP64 0
+#define WAMP_PREC_MIXED 1
+
+/* Create a context for one grid/EOS/scheme; allocates the step's scratch
+   (face fluxes, two stage buffers, dt accumulator) on the current device.
+   Replaces main()'s allocation of the 47 work arrays, SC:1839-1866. */
+int wamp_ctx_create(const wamp_grid *grid, const wamp_eos *eos, const wamp_scheme *scheme,
+                    int32_t precision, wamp_ctx **out);
+int wamp_ctx_destroy(wamp_ctx *ctx);
+const char *wamp_last_error(void);
+
+/* Bytes of device scratch the context holds. */
+int64_t wamp_ctx_scratch_bytes(const wamp_ctx *ctx);
+
+/* apply_bc, SC:1899-1905: fill the g ghost layers of U (and of the
+   primitive arrays P = [u, v, p] when P is non-NULL) from the interior. */
+int wamp_apply_bc(wamp_ctx *ctx, double *const U[6], double *const P[3], void *stream);
+
+/* update_prims, SC:1913-1917 (cons_to_prim_kernel SC:840-864) over the
+   padded grid: clamp a1 to [1e-8, 1-1e-8] in place, write u, v, p. */
+int wamp_cons_to_prim(wamp_ctx *ctx, double *const U[6], double *const P[3], void *stream);
+
+/* enforce_admissible, SC:1907-1911 (enforce_admissible_kernel SC:918-958),
+   over the padded grid.  A no-op when scheme.repair == 0. */
+int wamp_enforce_admissible(wamp_ctx *ctx, double *const U[6], void *stream);
+
+/* The initial sequence SC:1998-2001 (enforce_admissible, apply_bc,
+   update_prims, apply_bc) on the interior: repair (if enabled) and clamp a1,
+   in place.  Ghost layers are left to wamp_apply_bc. */
+int wamp_prepare(wamp_ctx *ctx, double *const U[6], void *stream);
+
+/* compute_fluxes + compute_low_fluxes, SC:1919-1947 (ducros_kernel,
+   fx_kernel, gy_kernel, fx_low_kernel, gy_low_kernel).  F receives 28
+   arrays of the state's shape in the reference's order: fx0..fx5, srcx,
+   gy0..gy5, srcy, fxl0..fxl5, srcxl, gyl0..gyl5, srcyl.  x-face values sit
+   at [i, j] for face i+1/2, i in [g-1, g+nx-1]; y-faces likewise.  The
+   boundary fill is applied on the fly; U's ghost layers are not read. */
+int wamp_compute_fluxes(wamp_ctx *ctx, double *const U[6], double *const F[28], void *stream);
+
+/* limit_fluxes + rk3_update + enforce_admissible + update_prims,
+   SC:1949-1984, 1907-1917: the three Wong stages, the SSP-RK3 combine
+   Uout = alpha*Un + beta*(U - dt*R(F)), repair and the a1 clamp, on the
+   interior.  Uout must not alias U (halo cells of U are read by
+   neighbouring tiles); it may alias Un.  dt is read from device memory.
+   When dt_acc is non-NULL the CFL minimum of the new state is folded into
+   *dt_acc (atomic min, timestep_kernel SC:1686-1705). */
+int wamp_limit_update(wamp_ctx *ctx, double *const Un[6], double *const U[6],
+                      double *const F[28], double *const Uout[6], double alpha, double beta,
+                      const double *dt_dev, double *dt_acc, void *stream);
+
+/* get_dt, SC:1986-1995 (timestep_kernel SC:1686-1705): *dt_dev =
+   cfl * min over interior cells of sx*sy/(sx+sy).  Device-resident; no
+   host synchronisation. */
+int wamp_get_dt(wamp_ctx *ctx, double *const U[6], double *dt_dev, void *stream);
+
+/* One SSP-RK3 step in place on U, SC:2028-2059: copy_old, three stages of
+   (apply_bc, fluxes, limiter, rk3_update, repair, prims), then the next dt.
+   dt is read from dt_dev; cfl * min of the new state is written to
+   dt_next_dev (may equal dt_dev).  U's ghost layers are unspecified on
+   return (call wamp_apply_bc for them). */
+int wamp_step(wamp_ctx *ctx, double *const U[6], const double *dt_dev, double *dt_next_dev,
+              void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

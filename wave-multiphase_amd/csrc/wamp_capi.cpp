@@ -1,0 +1,288 @@
+// wamp_capi.cpp -- C-ABI of libwamp.so (declared in include/wamp.h).
+//
+// Host side of the drop-in boundary: argument checks, scratch ownership and
+// the stage sequence of one SSP-RK3 step (SC:2028-2059).  No entry point
+// synchronises the device or falls back to a host computation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "wamp.h"
+
+namespace wamp {
+struct Geo {
+  int nx, ny, g, ld;
+  double dx, dy;
+  int bc[4];
+};
+struct Eos {
+  double g1, p1, g2, p2;
+};
+struct State {
+  double* f[6];
+};
+struct Faces {
+  double* f[28];
+};
+}  // namespace wamp
+
+using namespace wamp;
+
+extern "C" {
+hipError_t wamp_launch_faces(const Geo*, const Eos*, const State*, const Faces*, int base,
+                             int shear, int face_pp, int vf_always, double t1, hipStream_t);
+hipError_t wamp_launch_limit_update(const Geo*, const Eos*, const State* Un, const State* U,
+                                    const Faces*, const State* Uo, double alpha, double beta,
+                                    const double* dt_dev, double* dt_acc, int repair,
+                                    hipStream_t);
+hipError_t wamp_launch_dt(const Geo*, const Eos*, const State*, double* dt_acc, double* dt_out,
+                          double cfl, hipStream_t);
+hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t);
+hipError_t wamp_launch_fill(double* p, double v, hipStream_t);
+hipError_t wamp_launch_prepare(const Geo*, const Eos*, const State*, int repair, hipStream_t);
+hipError_t wamp_launch_enforce(const Geo*, const Eos*, const State*, hipStream_t);
+hipError_t wamp_launch_cons_to_prim(const Geo*, const Eos*, const State*, double*, double*,
+                                    double*, hipStream_t);
+hipError_t wamp_launch_bc(const Geo*, const State*, double*, double*, double*, hipStream_t);
+}
+
+struct wamp_ctx {
+  Geo geo;
+  Eos eos;
+  wamp_scheme scheme;
+  int precision;
+  int device;
+  double t1;          // tanh(beta_thinc / 2), SC:334
+  double* block;      // one allocation: faces | stage buffers | dt accumulator
+  int64_t bytes;
+  Faces faces;
+  State u1, u2;
+  double* dt_acc;
+};
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+static int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+static int state_of(double* const U[6], State& s, const char* name) {
+  if (!U) return fail(-2, "%s: NULL array list", name);
+  for (int m = 0; m < 6; ++m) {
+    if (!U[m]) return fail(-2, "%s[%d] is NULL", name, m);
+    s.f[m] = U[m];
+  }
+  return 0;
+}
+
+static int check_ctx(const wamp_ctx* c) {
+  if (!c) return fail(-1, "NULL context");
+  return 0;
+}
+
+extern "C" {
+
+const char* wamp_last_error(void) { return g_err.c_str(); }
+
+int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_scheme* scheme,
+                    int32_t precision, wamp_ctx** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
+  if (grid->nx < 1 || grid->ny < 1) return fail(-3, "nx, ny must be >= 1 (got %d, %d)", grid->nx, grid->ny);
+  if (grid->g < 4) return fail(-3, "g must be >= 4 (the face stencil reaches 4 cells), got %d", grid->g);
+  if (grid->ld < grid->ny + 2 * grid->g)
+    return fail(-3, "ld (%d) must be >= ny + 2g (%d)", grid->ld, grid->ny + 2 * grid->g);
+  if (!(grid->dx > 0.0) || !(grid->dy > 0.0)) return fail(-3, "dx, dy must be > 0");
+  if ((int64_t)grid->nx * grid->ny > (int64_t)1 << 30) return fail(-3, "grid too large");
+  for (int k = 0; k < 4; ++k)
+    if (grid->bc[k] != 0 && grid->bc[k] != 1) return fail(-3, "bc[%d] must be 0 or 1", k);
+  if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  if (scheme->base < 0 || scheme->base > 1) return fail(-3, "scheme.base must be 0 or 1");
+  if (scheme->shear < 0 || scheme->shear > 2) return fail(-3, "scheme.shear must be 0..2");
+  if (scheme->vf_thinc < 0 || scheme->vf_thinc > 1) return fail(-3, "scheme.vf_thinc must be 0 or 1");
+  if (scheme->face_pp < 0 || scheme->face_pp > 1) return fail(-3, "scheme.face_pp must be 0 or 1");
+  if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
+  if (precision != WAMP_PREC_FP64) return fail(-4, "precision %d is not available", precision);
+
+  wamp_ctx* c = new (std::nothrow) wamp_ctx();
+  if (!c) return fail(-5, "host allocation failed");
+  c->geo.nx = grid->nx;
+  c->geo.ny = grid->ny;
+  c->geo.g = grid->g;
+  c->geo.ld = grid->ld;
+  c->geo.dx = grid->dx;
+  c->geo.dy = grid->dy;
+  for (int k = 0; k < 4; ++k) c->geo.bc[k] = grid->bc[k];
+  c->eos = Eos{eos->gamma1, eos->pinf1, eos->gamma2, eos->pinf2};
+  c->scheme = *scheme;
+  c->precision = precision;
+  c->t1 = std::tanh(1.8 * 0.5);
+  int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
+  if (rc) { delete c; return rc; }
+
+  const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
+  const size_t n_planes = 28 + 12;
+  const size_t bytes = (n_planes * plane + 32) * sizeof(double);
+  rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(scratch)");
+  if (rc) { delete c; return rc; }
+  c->bytes = (int64_t)bytes;
+  double* p = c->block;
+  for (int m = 0; m < 28; ++m, p += plane) c->faces.f[m] = p;
+  for (int m = 0; m < 6; ++m, p += plane) c->u1.f[m] = p;
+  for (int m = 0; m < 6; ++m, p += plane) c->u2.f[m] = p;
+  c->dt_acc = p;
+  rc = hip_check(wamp_launch_fill(c->dt_acc, 1.0e30, nullptr), "fill dt accumulator");
+  if (!rc) rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  if (rc) {
+    (void)hipFree(c->block);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int wamp_ctx_destroy(wamp_ctx* c) {
+  if (!c) return 0;
+  int rc = 0;
+  if (c->block) rc = hip_check(hipFree(c->block), "hipFree(scratch)");
+  delete c;
+  return rc;
+}
+
+int64_t wamp_ctx_scratch_bytes(const wamp_ctx* c) { return c ? c->bytes : -1; }
+
+int wamp_apply_bc(wamp_ctx* c, double* const U[6], double* const P[3], void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  double *up = nullptr, *vp = nullptr, *pp = nullptr;
+  if (P) {
+    if (!P[0] || !P[1] || !P[2]) return fail(-2, "P has a NULL entry");
+    up = P[0]; vp = P[1]; pp = P[2];
+  }
+  return hip_check(wamp_launch_bc(&c->geo, &s, up, vp, pp, (hipStream_t)stream), "apply_bc");
+}
+
+int wamp_cons_to_prim(wamp_ctx* c, double* const U[6], double* const P[3], void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  if (!P || !P[0] || !P[1] || !P[2]) return fail(-2, "P must hold three non-NULL arrays");
+  return hip_check(wamp_launch_cons_to_prim(&c->geo, &c->eos, &s, P[0], P[1], P[2],
+                                            (hipStream_t)stream), "cons_to_prim");
+}
+
+int wamp_enforce_admissible(wamp_ctx* c, double* const U[6], void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  if (!c->scheme.repair) return 0;
+  return hip_check(wamp_launch_enforce(&c->geo, &c->eos, &s, (hipStream_t)stream),
+                   "enforce_admissible");
+}
+
+int wamp_prepare(wamp_ctx* c, double* const U[6], void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  return hip_check(wamp_launch_prepare(&c->geo, &c->eos, &s, c->scheme.repair, (hipStream_t)stream),
+                   "prepare");
+}
+
+static int launch_faces(wamp_ctx* c, const State& s, const Faces& f, hipStream_t st) {
+  return hip_check(wamp_launch_faces(&c->geo, &c->eos, &s, &f, c->scheme.base, c->scheme.shear,
+                                     c->scheme.face_pp, c->scheme.vf_thinc, c->t1, st),
+                   "faces");
+}
+
+int wamp_compute_fluxes(wamp_ctx* c, double* const U[6], double* const F[28], void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  if (!F) return fail(-2, "F is NULL");
+  Faces f;
+  for (int m = 0; m < 28; ++m) {
+    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
+    f.f[m] = F[m];
+  }
+  return launch_faces(c, s, f, (hipStream_t)stream);
+}
+
+int wamp_limit_update(wamp_ctx* c, double* const Un[6], double* const U[6], double* const F[28],
+                      double* const Uout[6], double alpha, double beta, const double* dt_dev,
+                      double* dt_acc, void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State sn, s, so;
+  if (int rc = state_of(Un, sn, "Un")) return rc;
+  if (int rc = state_of(U, s, "U")) return rc;
+  if (int rc = state_of(Uout, so, "Uout")) return rc;
+  for (int m = 0; m < 6; ++m)
+    if (so.f[m] == s.f[m]) return fail(-2, "Uout[%d] aliases U[%d]", m, m);
+  if (!F || !dt_dev) return fail(-2, "F and dt_dev must be non-NULL");
+  Faces f;
+  for (int m = 0; m < 28; ++m) {
+    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
+    f.f[m] = F[m];
+  }
+  return hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &sn, &s, &f, &so, alpha, beta, dt_dev,
+                                            dt_acc, c->scheme.repair, (hipStream_t)stream),
+                   "limit_update");
+}
+
+int wamp_get_dt(wamp_ctx* c, double* const U[6], double* dt_dev, void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  if (!dt_dev) return fail(-2, "dt_dev is NULL");
+  return hip_check(wamp_launch_dt(&c->geo, &c->eos, &s, c->dt_acc, dt_dev, c->scheme.cfl,
+                                  (hipStream_t)stream), "get_dt");
+}
+
+int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_next_dev,
+              void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  State a;
+  if (int rc = state_of(U, a, "U")) return rc;
+  if (!dt_dev || !dt_next_dev) return fail(-2, "dt_dev and dt_next_dev must be non-NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const int rep = c->scheme.repair;
+  // stage 1: U1 = Un + dt R(Un)            (SC:2031-2037)
+  if (int rc = launch_faces(c, a, c->faces, st)) return rc;
+  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &a, &c->faces, &c->u1, 0.0,
+                                                  1.0, dt_dev, nullptr, rep, st), "stage 1"))
+    return rc;
+  // stage 2: U2 = 3/4 Un + 1/4 (U1 + dt R(U1))   (SC:2040-2046)
+  if (int rc = launch_faces(c, c->u1, c->faces, st)) return rc;
+  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &c->u1, &c->faces, &c->u2,
+                                                  0.75, 0.25, dt_dev, nullptr, rep, st), "stage 2"))
+    return rc;
+  // stage 3: Un+1 = 1/3 Un + 2/3 (U2 + dt R(U2)), written over Un, with the
+  // CFL minimum of the new state folded in   (SC:2049-2059)
+  if (int rc = launch_faces(c, c->u2, c->faces, st)) return rc;
+  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &c->u2, &c->faces, &a,
+                                                  1.0 / 3.0, 2.0 / 3.0, dt_dev, c->dt_acc, rep, st),
+                         "stage 3"))
+    return rc;
+  return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, c->scheme.cfl, st), "dt");
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// wamp_device.h -- device math of the five-equation stiffened-gas step (gfx950).
+//
+// Scalar building blocks of the fused face-flux and limiter kernels.  Every
+// arithmetic expression keeps the reference's evaluation order (Python's
+// left-to-right association) so the FP64 path reproduces it to round-off;
+// the FP64 build compiles with contraction off.  Citations are to the
+// reference's Warp functions: SC = sainath_SC_it_works.py,
+// M10 = Sainath_M10.py.
+//
+// Design points that differ from the reference (exact, order-preserving):
+//  * the HLLC flux and its contact velocity u* share one evaluation of the
+//    wave speeds (hllc_x SC:594-662 and hllc_x_src SC:664-687 recompute the
+//    same cL, cR, SL, SR, SP from the same arguments);
+//  * face constants (u_n/2c, Y_k/c^2, ...) of the characteristic projection
+//    are formed once per face instead of once per stencil cell;
+//  * per-cell quantities (pressure, entropy, indicator, Ducros sensor) are
+//    formed once per cell by the callers, not once per face.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wamp {
+
+struct Eos {
+  double g1, p1, g2, p2;
+};
+
+// Warp builtin semantics: comparison selects, sign(0) = +1.
+template <typename T> __device__ __forceinline__ T wmin(T a, T b) { return a < b ? a : b; }
+template <typename T> __device__ __forceinline__ T wmax(T a, T b) { return a > b ? a : b; }
+template <typename T> __device__ __forceinline__ T wsign(T x) { return x < T(0) ? T(-1) : T(1); }
+template <typename T> __device__ __forceinline__ bool isnan_(T x) { return __builtin_isnan(x); }
+
+// Reference constants, SC:44-70.
+constexpr double TINY = 1.0e-20;
+constexpr double EPSM = 1.0e-20;
+constexpr double MP5C = 4.0;
+constexpr double B2 = 4.0 / 3.0;
+constexpr double KAP = 1.0 / 3.0;
+constexpr double ONE_60 = 1.0 / 60.0;
+constexpr double YUXIN = 0.9 * 0.35 * 1.0e-2 / (1.0 - 0.9 * 0.35);
+constexpr double THINC_THRESH = 0.35;
+constexpr double PP_RHO = 1.0e-10;
+constexpr double PP_ALPHA = 1.0e-10;
+constexpr double PP_CTILDE = 1.0e-8;
+constexpr double PP_HS_RHO = 1.0e-11;
+constexpr double PP_HS_ALPHA = 1.0e-11;
+constexpr double PP_HS_CTILDE = 1.0e-9;
+constexpr double SIGMA_SMOOTH = 0.01;
+constexpr double AL_CLIP = 1.0e-8;  // cons_to_prim clamp, SC:852-853
+
+// ---------------------------------------------------------------- EOS ------
+// mix_gamma_pibig SC:83-88
+template <typename T>
+__device__ __forceinline__ void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
+  T a2 = T(1) - a1;
+  gb = a1 / (T(e.g1) - T(1)) + a2 / (T(e.g2) - T(1));
+  pb = a1 * T(e.g1) * T(e.p1) / (T(e.g1) - T(1)) + a2 * T(e.g2) * T(e.p2) / (T(e.g2) - T(1));
+}
+// mix_gamma_pinf SC:90-95
+template <typename T>
+__device__ __forceinline__ void mix_g_pi(const Eos& e, T a1, T& g, T& pi) {
+  T gb, pb;
+  mix_gb_pb(e, a1, gb, pb);
+  g = T(1) / gb + T(1);
+  pi = (g - T(1)) * pb / g;
+}
+// sound_speed SC:97-103
+template <typename T>
+__device__ __forceinline__ T sound_speed(const Eos& e, T rho, T p, T a1) {
+  T g, pi;
+  mix_g_pi(e, a1, g, pi);
+  T val = g * (p + pi) / rho;
+  if (val < T(TINY)) val = T(TINY);
+  return sqrt(val);
+}
+// prim_to_energy SC:105-110
+template <typename T>
+__device__ __forceinline__ T prim_energy(const Eos& e, T ar1, T ar2, T u, T v, T p, T a1) {
+  T rho = ar1 + ar2;
+  T gb, pb;
+  mix_gb_pb(e, a1, gb, pb);
+  return gb * p + pb + T(0.5) * rho * (u * u + v * v);
+}
+// clamp01_pp SC:112-121
+template <typename T>
+__device__ __forceinline__ T clamp01_pp(T a1) {
+  T a = a1;
+  if (isnan_(a)) a = T(PP_ALPHA);
+  if (a < T(PP_ALPHA)) a = T(PP_ALPHA);
+  if (a > T(1) - T(PP_ALPHA)) a = T(1) - T(PP_ALPHA);
+  return a;
+}
+// pressure recovery from the conserved state, cons_to_prim_kernel SC:856-860
+// (always FP64: p is the small difference of E and the p_inf term).
+__device__ __forceinline__ double pressure(const Eos& e, double ar1, double ar2, double ru,
+                                           double rv, double E, double a1, double& uu,
+                                           double& vv) {
+  double rho = ar1 + ar2;
+  uu = ru / rho;
+  vv = rv / rho;
+  double gb, pb;
+  mix_gb_pb(e, a1, gb, pb);
+  return (E - pb - 0.5 * rho * (uu * uu + vv * vv)) / gb;
+}
+
+// ------------------------------------------------------- positivity --------
+// pp_theta_lower SC:123-138
+__device__ __forceinline__ double theta_lower(double base, double high, double floor_) {
+  double theta = 1.0;
+  if (isnan_(base) || isnan_(high)) {
+    theta = 0.0;
+  } else if (high < floor_) {
+    double denom = base - high;
+    theta = denom > TINY ? (base - floor_) / denom : 0.0;
+  }
+  if (theta < 0.0) theta = 0.0;
+  if (theta > 1.0) theta = 1.0;
+  return theta;
+}
+// rho_ctilde2_cons SC:242-258
+__device__ __forceinline__ double rho_ct2_cons(const Eos& e, double m1, double m2, double mx,
+                                               double my, double E, double a1_in) {
+  double a1 = clamp01_pp(a1_in);
+  if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
+  if (m2 < PP_RHO || isnan_(m2)) m2 = PP_RHO;
+  double rho = m1 + m2;
+  double uu = mx / rho;
+  double vv = my / rho;
+  double kin = 0.5 * rho * (uu * uu + vv * vv);
+  double g, pi;
+  mix_g_pi(e, a1, g, pi);
+  return E - kin - pi;
+}
+// theta_cons_stage SC:268-292, specialised per stage
+template <int STAGE>
+__device__ __forceinline__ double theta_stage(const Eos& e, const double* l, const double* h) {
+  double theta = 1.0;
+  if (STAGE == 0) {
+    theta = wmin(theta, theta_lower(l[0], h[0], PP_RHO));
+    theta = wmin(theta, theta_lower(l[1], h[1], PP_RHO));
+  } else if (STAGE == 1) {
+    theta = wmin(theta, theta_lower(l[5], h[5], PP_ALPHA));
+    theta = wmin(theta, theta_lower(1.0 - l[5], 1.0 - h[5], PP_ALPHA));
+  } else {
+    double lc = rho_ct2_cons(e, l[0], l[1], l[2], l[3], l[4], l[5]);
+    double hc = rho_ct2_cons(e, h[0], h[1], h[2], h[3], h[4], h[5]);
+    theta = theta_lower(lc, hc, PP_CTILDE);
+  }
+  if (theta < 0.0 || isnan_(theta)) theta = 0.0;
+  if (theta > 1.0) theta = 1.0;
+  return theta;
+}
+
+// ------------------------------------------------- reconstruction ----------
+template <typename T>
+__device__ __forceinline__ T mm2(T x, T y) {  // SC:295-297
+  return T(0.5) * (wsign(x) + wsign(y)) * wmin(fabs(x), fabs(y));
+}
+template <typename T>
+__device__ __forceinline__ T mm4(T a, T b, T c, T d) {  // SC:299-304
+  T s = wsign(a) + wsign(b);
+  return T(0.125) * s * fabs((wsign(a) + wsign(c)) * (wsign(a) + wsign(d))) *
+         wmin(wmin(fabs(a), fabs(b)), wmin(fabs(c), fabs(d)));
+}
+// cell_entropy SC:308-314
+__device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, double p, double a1) {
+  double rho = ar1 + ar2;
+  if (rho < TINY) rho = TINY;
+  double g, pi;
+  mix_g_pi(e, a1, g, pi);
+  return p / pow(rho, g);
+}
+// cator2_val SC:318-325
+__device__ __forceinline__ double cator2(double em2, double em1, double e0, double ep1, double ep2) {
+  double aa = 0.25 * fabs(em2 - 4.0 * em1 + 3.0 * e0) + (13.0 / 12.0) * fabs(e0 - 2.0 * em1 + em2);
+  double bb = 0.25 * fabs(3.0 * e0 - 4.0 * ep1 + ep2) + (13.0 / 12.0) * fabs(e0 - 2.0 * ep1 + ep2);
+  return (2.0 * aa * bb + YUXIN) / (aa * aa + bb * bb + YUXIN);
+}
+// thinc_ul SC:327-337; t1 = tanh(beta/2) is a per-launch constant
+template <typename T>
+__device__ __forceinline__ T thinc_l(T vm1, T v0, T v1, T t1) {
+  T result = v0;
+  if ((v1 - v0) * (v0 - vm1) > T(0)) {
+    T qa = T(0.5) * (v1 + vm1);
+    T qd = T(0.5) * (v1 - vm1);
+    T feng = (v0 - qa) / qd;
+    T t2 = tanh(feng * T(1.8) * T(0.5));
+    result = qa + qd * (t1 + t2 / t1) / (T(1) + t2);
+  }
+  return result;
+}
+// thinc_ur SC:339-349
+template <typename T>
+__device__ __forceinline__ T thinc_r(T v0, T v1, T v2, T t1) {
+  T result = v1;
+  if ((v2 - v1) * (v1 - v0) > T(0)) {
+    T qa = T(0.5) * (v2 + v0);
+    T qd = T(0.5) * (v2 - v0);
+    T feng = (v1 - qa) / qd;
+    T t2 = tanh(feng * T(1.8) * T(0.5));
+    result = qa - qd * (t1 - t2 / t1) / (T(1) - t2);
+  }
+  return result;
+}
+// mp5_ul SC:353-378 (the blend of SC:371-375 is overwritten by SC:376)
+template <typename T>
+__device__ __forceinline__ T mp5_l(T vm2, T vm1, T v0, T vp1, T vp2) {
+  T VOR = T(ONE_60) * (T(2) * vm2 - T(13) * vm1 + T(47) * v0 + T(27) * vp1 - T(3) * vp2);
+  T VMP = v0 + mm2(vp1 - v0, T(MP5C) * (v0 - vm1));
+  T result = VOR;
+  if ((VOR - v0) * (VOR - VMP) >= T(EPSM)) {
+    T DJM1 = vm2 - T(2) * vm1 + v0;
+    T DJ = vm1 - T(2) * v0 + vp1;
+    T DJP1 = v0 - T(2) * vp1 + vp2;
+    T DM4JPH = mm4(T(4) * DJ - DJP1, T(4) * DJP1 - DJ, DJ, DJP1);
+    T DM4JMH = mm4(T(4) * DJ - DJM1, T(4) * DJM1 - DJ, DJ, DJM1);
+    T VUL = v0 + T(MP5C) * (v0 - vm1);
+    T VAV = T(0.5) * (v0 + vp1);
+    T VMD = VAV - T(0.5) * DM4JPH;
+    T VLC = v0 + T(0.5) * (v0 - vm1) + T(B2) * DM4JMH;
+    T VMIN = wmax(wmin(wmin(v0, vp1), VMD), wmin(wmin(v0, VUL), VLC));
+    T VMAX = wmin(wmax(wmax(v0, vp1), VMD), wmax(wmax(v0, VUL), VLC));
+    result = VOR + mm2(VMIN - VOR, VMAX - VOR);
+  }
+  return result;
+}
+// mp5_ur SC:380-405
+template <typename T>
+__device__ __forceinline__ T mp5_r(T vm1, T v0, T vp1, T vp2, T vp3) {
+  T VOR = T(ONE_60) * (T(-3) * vm1 + T(27) * v0 + T(47) * vp1 - T(13) * vp2 + T(2) * vp3);
+  T VMP = vp1 + mm2(v0 - vp1, T(MP5C) * (vp1 - vp2));
+  T result = VOR;
+  if ((VOR - vp1) * (VOR - VMP) >= T(EPSM)) {
+    T DJM1 = vm1 - T(2) * v0 + vp1;
+    T DJ = v0 - T(2) * vp1 + vp2;
+    T DJP1 = vp1 - T(2) * vp2 + vp3;
+    T DM4JPH = mm4(T(4) * DJ - DJP1, T(4) * DJP1 - DJ, DJ, DJP1);
+    T DM4JMH = mm4(T(4) * DJ - DJM1, T(4) * DJM1 - DJ, DJ, DJM1);
+    T VUL = vp1 + T(MP5C) * (vp1 - vp2);
+    T VAV = T(0.5) * (vp1 + v0);
+    T VMD = VAV - T(0.5) * DM4JMH;
+    T VLC = vp1 + T(0.5) * (vp1 - vp2) + T(B2) * DM4JPH;
+    T VMIN = wmax(wmin(wmin(vp1, v0), VMD), wmin(wmin(vp1, VUL), VLC));
+    T VMAX = wmin(wmax(wmax(vp1, v0), VMD), wmax(wmax(vp1, VUL), VLC));
+    result = VOR + mm2(VMIN - VOR, VMAX - VOR);
+  }
+  return result;
+}
+// central6_avg SC:407-412
+template <typename T>
+__device__ __forceinline__ T central6(T vm2, T vm1, T v0, T vp1, T vp2, T vp3) {
+  T left = T(ONE_60) * (T(2) * vm2 - T(13) * vm1 + T(47) * v0 + T(27) * vp1 - T(3) * vp2);
+  T right = T(ONE_60) * (T(-3) * vm1 + T(27) * v0 + T(47) * vp1 - T(13) * vp2 + T(2) * vp3);
+  return T(0.5) * (left + right);
+}
+// 4th-order central interface value, SC:1077-1080
+template <typename T>
+__device__ __forceinline__ T central4(T vm1, T v0, T vp1, T vp2) {
+  return (-T(1.0 / 12.0) * vm1 + T(7.0 / 12.0) * v0 + T(7.0 / 12.0) * vp1 - T(1.0 / 12.0) * vp2);
+}
+// muscl_char_ul / muscl_char_ur SC:430-446
+template <typename T>
+__device__ __forceinline__ T muscl_l(T vm1, T v0, T vp1) {
+  T dm = v0 - vm1;
+  T d0 = vp1 - v0;
+  return v0 + T(0.25) * ((T(1) - T(KAP)) * mm2(dm, T(2) * d0) + (T(1) + T(KAP)) * mm2(d0, T(2) * dm));
+}
+template <typename T>
+__device__ __forceinline__ T muscl_r(T v0, T vp1, T vp2) {
+  T dp = vp2 - vp1;
+  T d0 = vp1 - v0;
+  return vp1 - T(0.25) * ((T(1) - T(KAP)) * mm2(dp, T(2) * d0) + (T(1) + T(KAP)) * mm2(d0, T(2) * dp));
+}
+
+// ---------------------------------------------- characteristic basis -------
+// Face-constant part of L and R (SC:462-478), formed once per face.
+template <typename T>
+struct Basis {
+  T c, k, inv2c, inv2c2, y1c2, y2c2, ut, Y1, Y2, un_, ut_, c2;
+};
+// DIR 0: x-face (normal u, SC:480-534); DIR 1: y-face (normal v, SC:537-589)
+template <int DIR, typename T>
+__device__ __forceinline__ Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg, T v_avg) {
+  Basis<T> b;
+  b.c = c;
+  b.inv2c = T(0.5) / c;
+  b.inv2c2 = T(0.5) / (c * c);
+  T Y1 = ar1 / rho;
+  T Y2 = ar2 / rho;
+  b.Y1 = Y1;
+  b.Y2 = Y2;
+  b.y1c2 = Y1 / (c * c);
+  b.y2c2 = Y2 / (c * c);
+  T un = DIR == 0 ? u_avg : v_avg;
+  b.k = un * b.inv2c;
+  b.ut = DIR == 0 ? v_avg : -u_avg;  // to_char_x: ut = v;  to_char_y: ut_f = -u
+  b.un_ = u_avg;
+  b.ut_ = v_avg;
+  b.c2 = c * c;
+  return b;
+}
+// L . u  (to_char_x SC:480-499 / to_char_y SC:537-556)
+template <int DIR, typename T>
+__device__ __forceinline__ void to_char(const Basis<T>& b, const T* u, T* w) {
+  const T un_m = DIR == 0 ? u[2] : u[3];  // normal momentum
+  const T ut_m = DIR == 0 ? u[3] : u[2];  // tangential momentum
+  w[0] = b.k * u[0] + b.k * u[1] - b.inv2c * un_m + b.inv2c2 * u[4];
+  w[1] = u[0] - b.y1c2 * u[4];
+  w[2] = u[1] - b.y2c2 * u[4];
+  if (DIR == 0)
+    w[3] = -b.ut * u[0] - b.ut * u[1] + ut_m;
+  else
+    w[3] = -b.ut * u[0] - b.ut * u[1] - ut_m;
+  w[4] = u[5];
+  w[5] = -b.k * u[0] - b.k * u[1] + b.inv2c * un_m + b.inv2c2 * u[4];
+}
+// R . w  (to_phys_x SC:504-534 / to_phys_y SC:559-589)
+template <int DIR, typename T>
+__device__ __forceinline__ void to_phys(const Basis<T>& b, const T* w, T* s) {
+  const T u_avg = b.un_, v_avg = b.ut_, c = b.c;
+  s[0] = b.Y1 * (w[0] + w[5]) + w[1];
+  s[1] = b.Y2 * (w[0] + w[5]) + w[2];
+  if (DIR == 0) {
+    s[2] = (u_avg - c) * w[0] + u_avg * w[1] + u_avg * w[2] + (u_avg + c) * w[5];
+    s[3] = v_avg * (w[0] + w[1] + w[2] + w[5]) + w[3];
+  } else {
+    s[2] = u_avg * (w[0] + w[1] + w[2] + w[5]) - w[3];
+    s[3] = (v_avg - c) * w[0] + v_avg * w[1] + v_avg * w[2] + (v_avg + c) * w[5];
+  }
+  s[4] = b.c2 * (w[0] + w[5]);
+  s[5] = w[4];
+}
+
+// ----------------------------------------------- Wong face limiter ---------
+// rho_ctilde2_sc SC:174-178
+__device__ __forceinline__ double rho_ct2_sc(const Eos& e, const double* s) {
+  double a1 = clamp01_pp(s[5]);
+  double g, pi;
+  mix_g_pi(e, a1, g, pi);
+  return (s[4] + pi) / (g - 1.0);
+}
+// wong_stage_lower_s6 SC:189-201 (in place on s)
+__device__ __forceinline__ void wong_lower(const double* b, double* s, double bv, double hv,
+                                           double floor_) {
+  double theta = 1.0;
+  if (isnan_(bv) || isnan_(hv) || bv < floor_) {
+    theta = 0.0;
+  } else if (hv < floor_) {
+    double denom = bv - hv;
+    theta = denom > TINY ? (bv - floor_) / denom : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) s[m] = b[m] + theta * (s[m] - b[m]);
+}
+// pp_limit_reconstructed_state SC:217-240 (in place on hi; base unchanged)
+__device__ __forceinline__ void pp_limit_state(const Eos& e, double* hi, const double* base) {
+  double b[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) b[m] = base[m];
+  b[5] = clamp01_pp(b[5]);
+  if (b[0] < PP_RHO || isnan_(b[0])) b[0] = PP_RHO;
+  if (b[1] < PP_RHO || isnan_(b[1])) b[1] = PP_RHO;
+  wong_lower(b, hi, b[0], hi[0], PP_RHO);
+  wong_lower(b, hi, b[1], hi[1], PP_RHO);
+  wong_lower(b, hi, b[5], hi[5], PP_ALPHA);
+  wong_lower(b, hi, 1.0 - b[5], 1.0 - hi[5], PP_ALPHA);
+  wong_lower(b, hi, rho_ct2_sc(e, b), rho_ct2_sc(e, hi), PP_CTILDE);
+  bool back = hi[0] < PP_HS_RHO || hi[1] < PP_HS_RHO || hi[5] < PP_HS_ALPHA ||
+              hi[5] > 1.0 - PP_HS_ALPHA;
+  if (back) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) hi[m] = b[m];
+  }
+  if (rho_ct2_sc(e, hi) < PP_HS_CTILDE || isnan_(hi[4])) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) hi[m] = b[m];
+  }
+}
+// pp_floor_pressure_state SC:180-187 (in place)
+__device__ __forceinline__ void floor_pressure(const Eos& e, double* s) {
+  double a1 = clamp01_pp(s[5]);
+  double g, pi;
+  mix_g_pi(e, a1, g, pi);
+  if (isnan_(s[4]) || rho_ct2_sc(e, s) < PP_CTILDE) s[4] = PP_CTILDE * (g - 1.0) - pi;
+  s[5] = a1;
+}
+
+// ---------------------------------------------------------------- HLLC -----
+// hllc_x + hllc_x_src (SC:594-687) / hllc_y + hllc_y_src (SC:690-771) fused:
+// one evaluation of cL, cR, SL, SR, SP feeds the flux and the contact
+// velocity u*.  Inputs are primitives (ar1, ar2, u, v, p, a1) per side.
+template <int DIR, typename T>
+__device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
+  const T ar1L = L[0], ar2L = L[1], uL = L[2], vL = L[3], pL = L[4], a1L = L[5];
+  const T ar1R = R[0], ar2R = R[1], uR = R[2], vR = R[3], pR = R[4], a1R = R[5];
+  const T nL = DIR == 0 ? uL : vL, nR = DIR == 0 ? uR : vR;
+  T rhoL = ar1L + ar2L;
+  T rhoR = ar1R + ar2R;
+  T cL = sound_speed(e, rhoL, pL, a1L);
+  T cR = sound_speed(e, rhoR, pR, a1R);
+  T EL = prim_energy(e, ar1L, ar2L, uL, vL, pL, a1L);
+  T ER = prim_energy(e, ar1R, ar2R, uR, vR, pR, a1R);
+  T n_avg = T(0.5) * (nL + nR);
+  T c_avg = T(0.5) * (cL + cR);
+  T SL = wmin(nL - cL, n_avg - c_avg);
+  T SR = wmax(nR + cR, n_avg + c_avg);
+  T denom = rhoL * (SL - nL) - rhoR * (SR - nR);
+  if (fabs(denom) < T(TINY)) denom = T(TINY);
+  T SP = (pR - pL + rhoL * nL * (SL - nL) - rhoR * nR * (SR - nR)) / denom;
+  T ratL = (SL - nL) / (SL - SP);
+  T ratR = (SR - nR) / (SR - SP);
+
+  // tangential momentum flux rho*u*v: x-face forms (rho*u)*v, y-face (rho*u)*v
+  // with u tangential -- both are rho*uL*vL in the reference's order.
+  if (SL > T(0)) {
+    f[0] = ar1L * nL;
+    f[1] = ar2L * nL;
+    if (DIR == 0) { f[2] = rhoL * uL * uL + pL; f[3] = rhoL * uL * vL; }
+    else          { f[2] = rhoL * uL * vL;      f[3] = rhoL * vL * vL + pL; }
+    f[4] = (EL + pL) * nL;
+    f[5] = a1L * nL;
+  } else if (SL <= T(0) && T(0) < SP) {
+    T ratio = ratL;
+    T EL_star = EL + (SP - nL) * (SP * rhoL + pL / (SL - nL));
+    f[0] = ar1L * nL + SL * (ar1L * ratio - ar1L);
+    f[1] = ar2L * nL + SL * (ar2L * ratio - ar2L);
+    if (DIR == 0) {
+      f[2] = (rhoL * uL * uL + pL) + SL * (rhoL * SP * ratio - rhoL * uL);
+      f[3] = (rhoL * uL * vL) + SL * (rhoL * vL * ratio - rhoL * vL);
+    } else {
+      f[2] = (rhoL * uL * vL) + SL * (rhoL * uL * ratio - rhoL * uL);
+      f[3] = (rhoL * vL * vL + pL) + SL * (rhoL * SP * ratio - rhoL * vL);
+    }
+    f[4] = (EL + pL) * nL + SL * (EL_star * ratio - EL);
+    f[5] = a1L * nL + SL * (a1L * ratio - a1L);
+  } else if (SP <= T(0) && T(0) <= SR) {
+    T ratio = ratR;
+    T ER_star = ER + (SP - nR) * (SP * rhoR + pR / (SR - nR));
+    f[0] = ar1R * nR + SR * (ar1R * ratio - ar1R);
+    f[1] = ar2R * nR + SR * (ar2R * ratio - ar2R);
+    if (DIR == 0) {
+      f[2] = (rhoR * uR * uR + pR) + SR * (rhoR * SP * ratio - rhoR * uR);
+      f[3] = (rhoR * uR * vR) + SR * (rhoR * vR * ratio - rhoR * vR);
+    } else {
+      f[2] = (rhoR * uR * vR) + SR * (rhoR * uR * ratio - rhoR * uR);
+      f[3] = (rhoR * vR * vR + pR) + SR * (rhoR * SP * ratio - rhoR * vR);
+    }
+    f[4] = (ER + pR) * nR + SR * (ER_star * ratio - ER);
+    f[5] = a1R * nR + SR * (a1R * ratio - a1R);
+  } else {
+    f[0] = ar1R * nR;
+    f[1] = ar2R * nR;
+    if (DIR == 0) { f[2] = rhoR * uR * uR + pR; f[3] = rhoR * uR * vR; }
+    else          { f[2] = rhoR * uR * vR;      f[3] = rhoR * vR * vR + pR; }
+    f[4] = (ER + pR) * nR;
+    f[5] = a1R * nR;
+  }
+  T left_part = nL + wmin(T(0), SL) * (ratL - T(1));
+  T right_part = nR + wmax(T(0), SR) * (ratR - T(1));
+  T sp_sign = wsign(SP);
+  ustar = T(0.5) * (T(1) + sp_sign) * left_part + T(0.5) * (T(1) - sp_sign) * right_part;
+}
+
+}  // namespace wamp

@@ -1,0 +1,746 @@
+// wamp_kernels.hip -- fused HIP kernels of the SSP-RK3 step for gfx950 (FP64).
+//
+// Per RK stage the reference launches ~20 kernels over 47 arrays
+// (SC:2031-2037).  Here a stage is two launches:
+//
+//   k_faces        one LDS tile of 8x32 face positions: loads the 6
+//                  conserved fields of the tile plus a 3/4-cell halo once
+//                  (the boundary fill SC:1616-1682 is applied by index
+//                  mapping while loading), forms pressure, entropy, the
+//                  entropy indicator and the Ducros sensor once per cell,
+//                  then every thread evaluates one x-face and one y-face,
+//                  high order (fx_kernel SC:966-1133 / M10:868-1045) and
+//                  first order (fx_low_kernel SC:1303-1347), and stores the
+//                  28 face arrays.
+//   k_limit_update one 16x16 cell tile with a 3-cell halo frame in LDS: the
+//                  three Wong theta stages (pp_cell_theta / pp_blend,
+//                  SC:1397-1548) on shrinking frames, the final blended
+//                  fluxes, the SSP-RK3 combine with the alpha-source term
+//                  (rk3_kernel SC:1553-1611), the emergency repair
+//                  (SC:918-958), the alpha clamp (SC:851-854) and, on the
+//                  last stage, the CFL minimum (SC:1686-1705).
+//
+// Compiled with -ffp-contract=off: every expression keeps the reference's
+// association, so the FP64 results agree with the reference's arithmetic
+// up to the ulp-level differences of pow/tanh between math libraries.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wamp_device.h"
+
+namespace wamp {
+
+struct Geo {
+  int nx, ny, g, ld;
+  double dx, dy;
+  int bc[4];
+};
+struct State {
+  double* f[6];
+};
+struct Faces {
+  double* f[28];
+};
+
+// ------------------------------------------------------------------------
+// Boundary fill by index mapping.  Ghost cell (i, j) holds the edge
+// interior cell: bc_x copies cell g / g+nx-1 into every x-ghost layer
+// (SC:1630-1646), then bc_y copies row g / g+ny-1 of the x-filled array
+// into every y-ghost layer (SC:1664-1682), so corner cells hold the corner
+// interior cell.  Slip walls negate the wall-normal momentum.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ size_t map_cell(const Geo& G, int i, int j, double& sru, double& srv) {
+  sru = 1.0;
+  srv = 1.0;
+  if (i < G.g) {
+    i = G.g;
+    if (G.bc[0]) sru = -1.0;
+  } else if (i >= G.g + G.nx) {
+    i = G.g + G.nx - 1;
+    if (G.bc[1]) sru = -1.0;
+  }
+  if (j < G.g) {
+    j = G.g;
+    if (G.bc[2]) srv = -1.0;
+  } else if (j >= G.g + G.ny) {
+    j = G.g + G.ny - 1;
+    if (G.bc[3]) srv = -1.0;
+  }
+  return (size_t)i * (size_t)G.ld + (size_t)j;
+}
+
+// ========================================================================
+// k_faces
+// ========================================================================
+constexpr int FTI = 8;             // face positions per tile along i
+constexpr int FTJ = 32;            // ... along j (contiguous)
+constexpr int FRI = FTI + 7;       // LDS rectangle: cells i0-3 .. i0+FTI+3
+constexpr int FRJ = FTJ + 7;
+constexpr int FRN = FRI * FRJ;
+constexpr int FCXN = (FTI + 3) * FTJ;  // x-indicator cells: i0-1..i0+FTI+1, j0..j0+FTJ-1
+constexpr int FCYN = FTI * (FTJ + 3);  // y-indicator cells: i0..i0+FTI-1, j0-1..j0+FTJ+1
+constexpr int FTHREADS = FTI * FTJ;
+
+struct FaceSmem {
+  double q[6][FRN];  // ar1, ar2, ru, rv, p, a1
+  double ent[FRN];
+  double cx[FCXN], cy[FCYN];
+  double sx[FCXN], sy[FCYN];
+};
+
+// Ducros x pressure sensor (ducros_kernel SC:875-916) at LDS cell r.
+__device__ __forceinline__ double lds_vel(const FaceSmem& S, int r, int comp) {
+  return S.q[comp][r] / (S.q[0][r] + S.q[1][r]);
+}
+template <int DIR>
+__device__ __forceinline__ double ducros_sigma(const FaceSmem& S, int r) {
+  double der_ux = 0.5 * (-lds_vel(S, r - FRJ, 2) + lds_vel(S, r + FRJ, 2));
+  double der_vx = 0.5 * (-lds_vel(S, r - FRJ, 3) + lds_vel(S, r + FRJ, 3));
+  double der_uy = 0.5 * (-lds_vel(S, r - 1, 2) + lds_vel(S, r + 1, 2));
+  double der_vy = 0.5 * (-lds_vel(S, r - 1, 3) + lds_vel(S, r + 1, 3));
+  double div_u = der_ux + der_vy;
+  double curl_u = der_vx - der_uy;
+  double ducros = div_u * div_u / (div_u * div_u + curl_u * curl_u + 1.0e-30);
+  const int s = DIR == 0 ? FRJ : 1;
+  const double* p = S.q[4];
+  double pm2 = p[r - 2 * s], pm1 = p[r - s], p0 = p[r], pp1 = p[r + s], pp2 = p[r + 2 * s];
+  double aa = fabs(-pm2 + 16.0 * pm1 - 30.0 * p0 + 16.0 * pp1 - pp2);
+  double bb = fabs(pm2 + 16.0 * pm1 + 30.0 * p0 + 16.0 * pp1 + pp2);
+  return (aa / (bb + 1.0e-30)) * ducros;
+}
+
+// High-order flux through one face whose left cell is LDS index r0; cells
+// along the normal are r0 + k*s.  ci / sg: indicator and Ducros sensor at
+// the cells -1..2.  Writes 6 fluxes + u* to out.
+template <int DIR, int BASE, int SHEAR, int FACEPP>
+__device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r0, const double* ci,
+                                          const double* sg, int vf_always, double t1, double* out) {
+  constexpr int s = DIR == 0 ? FRJ : 1;
+  constexpr bool wide = BASE == 1 || SHEAR == 2;  // needs cells -2 and +3
+  constexpr int klo = wide ? 0 : 1, khi = wide ? 6 : 5;
+
+  bool use_thinc = wmin(wmin(ci[0], ci[1]), wmin(ci[2], ci[3])) < THINC_THRESH;
+  bool use_smooth = false;
+  if (SHEAR != 0) use_smooth = wmax(wmax(sg[0], sg[1]), wmax(sg[2], sg[3])) < SIGMA_SMOOTH;
+
+  // face-averaged state, SC:1039-1048
+  const int rA = r0, rB = r0 + s;
+  double ar1a = 0.5 * (S.q[0][rA] + S.q[0][rB]);
+  double ar2a = 0.5 * (S.q[1][rA] + S.q[1][rB]);
+  double a1a = 0.5 * (S.q[5][rA] + S.q[5][rB]);
+  double pa = 0.5 * (S.q[4][rA] + S.q[4][rB]);
+  double rhoa = ar1a + ar2a;
+  double u_avg = 0.5 * (S.q[2][rA] + S.q[2][rB]) / rhoa;
+  double v_avg = 0.5 * (S.q[3][rA] + S.q[3][rB]) / rhoa;
+  double ca = sound_speed(e, rhoa, pa, a1a);
+  Basis<double> B = make_basis<DIR>(ca, rhoa, ar1a, ar2a, u_avg, v_avg);
+
+  // characteristic projection of the stencil cells -2..3 (SC:1051-1056)
+  double w[6][6];
+#pragma unroll
+  for (int k = klo; k < khi; ++k) {
+    double u[6];
+    const int r = r0 + (k - 2) * s;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = S.q[m][r];
+    to_char<DIR>(B, u, w[k]);
+  }
+  double wl[6], wr[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    if (BASE == 0) {
+      wl[m] = muscl_l(w[1][m], w[2][m], w[3][m]);
+      wr[m] = muscl_r(w[2][m], w[3][m], w[4][m]);
+    } else {
+      wl[m] = mp5_l(w[0][m], w[1][m], w[2][m], w[3][m], w[4][m]);
+      wr[m] = mp5_r(w[1][m], w[2][m], w[3][m], w[4][m], w[5][m]);
+    }
+  }
+  if (SHEAR != 0 && use_smooth) {
+    double iph = SHEAR == 1 ? central4(w[1][3], w[2][3], w[3][3], w[4][3])
+                            : central6(w[0][3], w[1][3], w[2][3], w[3][3], w[4][3], w[5][3]);
+    wl[3] = iph;
+    wr[3] = iph;
+  }
+  if (use_thinc) {
+    wl[1] = thinc_l(w[1][1], w[2][1], w[3][1], t1);
+    wl[2] = thinc_l(w[1][2], w[2][2], w[3][2], t1);
+    wr[1] = thinc_r(w[2][1], w[3][1], w[4][1], t1);
+    wr[2] = thinc_r(w[2][2], w[3][2], w[4][2], t1);
+  }
+  if (use_thinc || vf_always) {
+    wl[4] = thinc_l(w[1][4], w[2][4], w[3][4], t1);
+    wr[4] = thinc_r(w[2][4], w[3][4], w[4][4], t1);
+  }
+  double phL[6], phR[6], phfL[6], phfR[6];
+  to_phys<DIR>(B, wl, phL);
+  to_phys<DIR>(B, wr, phR);
+  to_phys<DIR>(B, w[2], phfL);
+  to_phys<DIR>(B, w[3], phfR);
+
+  if (FACEPP == 0) {  // SC:1103-1111
+    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
+    if (isnan_(phL[4])) phL[4] = phfL[4];
+    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
+    if (isnan_(phR[4])) phR[4] = phfR[4];
+  } else {  // M10:1007-1021
+    pp_limit_state(e, phL, phfL);
+    pp_limit_state(e, phR, phfR);
+    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
+    if (rho_ct2_sc(e, phL) < PP_CTILDE || isnan_(phL[4])) phL[4] = phfL[4];
+    floor_pressure(e, phL);
+    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
+    if (rho_ct2_sc(e, phR) < PP_CTILDE || isnan_(phR[4])) phR[4] = phfR[4];
+    floor_pressure(e, phR);
+  }
+  if (ca < 0.0 || isnan_(ca)) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      phL[m] = phfL[m];
+      phR[m] = phfR[m];
+    }
+  }
+  // primitives for HLLC, SC:1118-1125
+  double rhoL = phL[0] + phL[1], rhoR = phR[0] + phR[1];
+  double PL[6] = {phL[0], phL[1], phL[2] / rhoL, phL[3] / rhoL, phL[4], phL[5]};
+  double PR[6] = {phR[0], phR[1], phR[2] / rhoR, phR[3] / rhoR, phR[4], phR[5]};
+  hllc<DIR>(e, PL, PR, out, out[6]);
+}
+
+// First-order flux (fx_low_kernel SC:1303-1347 / gy_low_kernel SC:1350-1394).
+template <int DIR>
+__device__ __forceinline__ void face_low(const Eos& e, const FaceSmem& S, int r0, double* out) {
+  constexpr int s = DIR == 0 ? FRJ : 1;
+  const int rL = r0, rR = r0 + s;
+  double rhoL = S.q[0][rL] + S.q[1][rL];
+  double rhoR = S.q[0][rR] + S.q[1][rR];
+  double uL = S.q[2][rL] / rhoL;
+  double vL = S.q[3][rL] / rhoL;
+  double uR = S.q[2][rR] / rhoR;
+  double vR = S.q[3][rR] / rhoR;
+  double aL = clamp01_pp(S.q[5][rL]);
+  double aR = clamp01_pp(S.q[5][rR]);
+  double pL = S.q[4][rL];
+  double pR = S.q[4][rR];
+  double gL, piL, gR, piR;
+  mix_g_pi(e, aL, gL, piL);
+  mix_g_pi(e, aR, gR, piR);
+  if (isnan_(pL) || (pL + piL) / (gL - 1.0) < PP_CTILDE) pL = PP_CTILDE * (gL - 1.0) - piL;
+  if (isnan_(pR) || (pR + piR) / (gR - 1.0) < PP_CTILDE) pR = PP_CTILDE * (gR - 1.0) - piR;
+  double PL[6] = {S.q[0][rL], S.q[1][rL], uL, vL, pL, aL};
+  double PR[6] = {S.q[0][rR], S.q[1][rR], uR, vR, pR, aR};
+  hllc<DIR>(e, PL, PR, out, out[6]);
+}
+
+template <int BASE, int SHEAR, int FACEPP>
+__global__ void __launch_bounds__(FTHREADS)
+k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
+  __shared__ FaceSmem S;
+  const int tid = threadIdx.x;
+  const int i0 = G.g - 1 + (int)blockIdx.y * FTI;
+  const int j0 = G.g - 1 + (int)blockIdx.x * FTJ;
+
+  // Phase A: stage the cells of the rectangle, pressure and entropy once.
+  for (int r = tid; r < FRN; r += FTHREADS) {
+    const int i = i0 - 3 + r / FRJ;
+    const int j = j0 - 3 + r % FRJ;
+    double sru, srv;
+    const size_t c = map_cell(G, i, j, sru, srv);
+    double ar1 = U.f[0][c], ar2 = U.f[1][c];
+    double ru = U.f[2][c], rv = U.f[3][c];
+    double E = U.f[4][c], al = U.f[5][c];
+    if (sru < 0.0) ru = -1.0 * ru;
+    if (srv < 0.0) rv = -1.0 * rv;
+    double uu, vv;
+    double p = pressure(e, ar1, ar2, ru, rv, E, al, uu, vv);
+    S.q[0][r] = ar1;
+    S.q[1][r] = ar2;
+    S.q[2][r] = ru;
+    S.q[3][r] = rv;
+    S.q[4][r] = p;
+    S.q[5][r] = al;
+    S.ent[r] = entropy(e, ar1, ar2, p, al);
+  }
+  __syncthreads();
+
+  // Phase B: entropy indicator (cator2_val SC:318-325) and Ducros sensor
+  // once per cell, along each sweep direction.
+  for (int k = tid; k < FCXN; k += FTHREADS) {
+    const int r = (k / FTJ + 2) * FRJ + (k % FTJ + 3);
+    S.cx[k] = cator2(S.ent[r - 2 * FRJ], S.ent[r - FRJ], S.ent[r], S.ent[r + FRJ], S.ent[r + 2 * FRJ]);
+    if (SHEAR != 0) S.sx[k] = ducros_sigma<0>(S, r);
+  }
+  for (int k = tid; k < FCYN; k += FTHREADS) {
+    const int r = (k / (FTJ + 3) + 3) * FRJ + (k % (FTJ + 3) + 2);
+    S.cy[k] = cator2(S.ent[r - 2], S.ent[r - 1], S.ent[r], S.ent[r + 1], S.ent[r + 2]);
+    if (SHEAR != 0) S.sy[k] = ducros_sigma<1>(S, r);
+  }
+  __syncthreads();
+
+  // Phase C: one x-face and one y-face per thread.
+  const int ti = tid / FTJ, tj = tid % FTJ;
+  const int i = i0 + ti, j = j0 + tj;
+  const int r0 = (ti + 3) * FRJ + (tj + 3);
+  const size_t o = (size_t)i * (size_t)G.ld + (size_t)j;
+  if (i >= G.g - 1 && i <= G.g + G.nx - 1 && j >= G.g && j <= G.g + G.ny - 1) {
+    double ci[4], sg[4], out[7];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ci[k] = S.cx[(ti + k) * FTJ + tj];
+      sg[k] = SHEAR != 0 ? S.sx[(ti + k) * FTJ + tj] : 0.0;
+    }
+    face_high<0, BASE, SHEAR, FACEPP>(e, S, r0, ci, sg, vf_always, t1, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F.f[m][o] = out[m];
+    face_low<0>(e, S, r0, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F.f[14 + m][o] = out[m];
+  }
+  if (i >= G.g && i <= G.g + G.nx - 1 && j >= G.g - 1 && j <= G.g + G.ny - 1) {
+    double ci[4], sg[4], out[7];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ci[k] = S.cy[ti * (FTJ + 3) + tj + k];
+      sg[k] = SHEAR != 0 ? S.sy[ti * (FTJ + 3) + tj + k] : 0.0;
+    }
+    face_high<1, BASE, SHEAR, FACEPP>(e, S, r0, ci, sg, vf_always, t1, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F.f[7 + m][o] = out[m];
+    face_low<1>(e, S, r0, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F.f[21 + m][o] = out[m];
+  }
+}
+
+// ========================================================================
+// k_limit_update
+// ========================================================================
+constexpr int LT = 16;             // cells per tile side
+constexpr int LH = 3;              // theta halo
+constexpr int LF = LT + 2 * LH;    // frame side (22)
+constexpr int LTHREADS = 256;
+
+// Blended flux after `nb` limiter stages: f <- L + theta_k (f - L), k < nb
+// (pp_blend_x_flux_kernel SC:1506-1512 applied once per stage).
+__device__ __forceinline__ double blend(double H, double L, const double* th, int nb) {
+  double f = H;
+  for (int k = 0; k < nb; ++k) f = L + th[k] * (f - L);
+  return f;
+}
+
+struct CellFaces {
+  // theta of the four faces for each applied stage: x-, x+, y-, y+
+  double t[4][3];
+};
+
+// Face theta from the two adjacent cells' theta (non-interior cells hold 1),
+// pp_blend_x/y SC:1500-1504, 1536-1540.
+__device__ __forceinline__ double face_theta(double ta, double tb) {
+  double th = 1.0;
+  th = wmin(th, ta);
+  th = wmin(th, tb);
+  return th;
+}
+
+// Residual of component m (0..5) at cell c from blended (rh) and low (rl)
+// face fluxes; m == 5 adds the alpha source with alpha = a (SC:1444-1460).
+__device__ __forceinline__ void residual(const Faces& F, int m, size_t c, size_t ld,
+                                         const CellFaces& cf, int nb, double inv_dx,
+                                         double inv_dy, double a, double& rh, double& rl) {
+  const double *fx = F.f[m], *gy = F.f[7 + m], *fxl = F.f[14 + m], *gyl = F.f[21 + m];
+  const size_t cx = c - ld, cy = c - 1;
+  double Lxp = fxl[c], Lxm = fxl[cx], Lyp = gyl[c], Lym = gyl[cy];
+  double Hxp = blend(fx[c], Lxp, cf.t[1], nb);
+  double Hxm = blend(fx[cx], Lxm, cf.t[0], nb);
+  double Hyp = blend(gy[c], Lyp, cf.t[3], nb);
+  double Hym = blend(gy[cy], Lym, cf.t[2], nb);
+  rh = (Hxp - Hxm) * inv_dx + (Hyp - Hym) * inv_dy;
+  rl = (Lxp - Lxm) * inv_dx + (Lyp - Lym) * inv_dy;
+  if (m == 5) {
+    const double *sx = F.f[6], *sy = F.f[13], *sxl = F.f[20], *syl = F.f[27];
+    double sLxp = sxl[c], sLxm = sxl[cx], sLyp = syl[c], sLym = syl[cy];
+    double sHxp = blend(sx[c], sLxp, cf.t[1], nb);
+    double sHxm = blend(sx[cx], sLxm, cf.t[0], nb);
+    double sHyp = blend(sy[c], sLyp, cf.t[3], nb);
+    double sHym = blend(sy[cy], sLym, cf.t[2], nb);
+    rh = rh - a * (sHxp - sHxm) * inv_dx - a * (sHyp - sHym) * inv_dy;
+    rl = rl - a * (sLxp - sLxm) * inv_dx - a * (sLyp - sLym) * inv_dy;
+  }
+}
+
+// enforce_admissible_kernel SC:918-958 on one cell (in registers).
+__device__ __forceinline__ void repair_cell(const Eos& e, double* u) {
+  double a1 = clamp01_pp(u[5]);
+  double m1 = u[0], m2 = u[1], mx = u[2], my = u[3], E = u[4];
+  if (isnan_(m1) || m1 < PP_RHO) m1 = PP_RHO;
+  if (isnan_(m2) || m2 < PP_RHO) m2 = PP_RHO;
+  if (isnan_(mx)) mx = 0.0;
+  if (isnan_(my)) my = 0.0;
+  double rho = m1 + m2;
+  double uu = mx / rho;
+  double vv = my / rho;
+  double kin = 0.5 * rho * (uu * uu + vv * vv);
+  double gb, pb;
+  mix_gb_pb(e, a1, gb, pb);
+  double p = (E - pb - kin) / gb;
+  double g, pi;
+  mix_g_pi(e, a1, g, pi);
+  double p_min = PP_CTILDE * (g - 1.0) - pi;
+  if (isnan_(E) || isnan_(p) || rho_ct2_cons(e, m1, m2, mx, my, E, a1) < PP_CTILDE)
+    E = gb * p_min + pb + kin;
+  u[0] = m1;
+  u[1] = m2;
+  u[2] = mx;
+  u[3] = my;
+  u[4] = E;
+  u[5] = a1;
+}
+
+__device__ __forceinline__ double clamp_al(double a1) {  // SC:852-853
+  if (a1 < AL_CLIP) a1 = AL_CLIP;
+  if (a1 > 1.0 - AL_CLIP) a1 = 1.0 - AL_CLIP;
+  return a1;
+}
+
+// timestep_kernel SC:1700-1704 on one cell whose a1 is already clamped.
+__device__ __forceinline__ double cell_dt(const Eos& e, const double* u, double dx, double dy) {
+  double uu, vv;
+  double p = pressure(e, u[0], u[1], u[2], u[3], u[4], u[5], uu, vv);
+  double rho = u[0] + u[1];
+  double c = sound_speed(e, rho, p, u[5]);
+  double sx = dx / (fabs(uu) + c);
+  double sy = dy / (fabs(vv) + c);
+  return sx * sy / (sx + sy);
+}
+
+// Block-wide min of a positive double, one atomic per block (bit pattern
+// order equals numeric order for non-negative doubles).
+__device__ __forceinline__ void block_min_atomic(double v, double* acc) {
+  __shared__ double wmins[LTHREADS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = wmin(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wmins[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmins[0];
+    for (int w = 1; w < LTHREADS / 64; ++w) m = wmin(m, wmins[w]);
+    atomicMin((unsigned long long*)acc, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void __launch_bounds__(LTHREADS)
+k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
+               const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+  __shared__ double th[3][LF * LF];
+  const int tid = threadIdx.x;
+  const int i0 = G.g + (int)blockIdx.y * LT;
+  const int j0 = G.g + (int)blockIdx.x * LT;
+  const size_t ld = (size_t)G.ld;
+  const double dt = *dt_dev;
+  const double inv_dx = 1.0 / G.dx;
+  const double inv_dy = 1.0 / G.dy;
+
+  auto interior = [&](int i, int j) {
+    return i >= G.g && i < G.g + G.nx && j >= G.g && j < G.g + G.ny;
+  };
+  auto faces_of = [&](int a, int b, int nb, CellFaces& cf) {
+    for (int k = 0; k < nb; ++k) {
+      const double* t = th[k];
+      const int r = a * LF + b;
+      cf.t[0][k] = face_theta(t[r - LF], t[r]);
+      cf.t[1][k] = face_theta(t[r], t[r + LF]);
+      cf.t[2][k] = face_theta(t[r - 1], t[r]);
+      cf.t[3][k] = face_theta(t[r], t[r + 1]);
+    }
+  };
+
+  // Three theta stages on frames of halo 3, 2, 1 (pp_cell_theta SC:1436-1476).
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int lo = s, n = LF - 2 * s;
+    for (int k = tid; k < n * n; k += LTHREADS) {
+      const int a = lo + k / n, b = lo + k % n;
+      const int i = i0 - LH + a, j = j0 - LH + b;
+      double theta = 1.0;
+      if (interior(i, j)) {
+        const size_t c = (size_t)i * ld + (size_t)j;
+        CellFaces cf;
+        faces_of(a, b, s, cf);
+        double al = U.f[5][c];
+        double l[6], h[6];
+        if (s == 0) {
+#pragma unroll
+          for (int m = 0; m < 2; ++m) {
+            double rh, rl;
+            residual(F, m, c, ld, cf, 0, inv_dx, inv_dy, al, rh, rl);
+            double u = U.f[m][c];
+            l[m] = u - dt * rl;
+            h[m] = u - dt * rh;
+          }
+          theta = theta_stage<0>(e, l, h);
+        } else if (s == 1) {
+          double rh, rl;
+          residual(F, 5, c, ld, cf, 1, inv_dx, inv_dy, al, rh, rl);
+          l[5] = al - dt * rl;
+          h[5] = al - dt * rh;
+          theta = theta_stage<1>(e, l, h);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) {
+            double rh, rl;
+            residual(F, m, c, ld, cf, 2, inv_dx, inv_dy, al, rh, rl);
+            double u = U.f[m][c];
+            l[m] = u - dt * rl;
+            h[m] = u - dt * rh;
+          }
+          theta = theta_stage<2>(e, l, h);
+        }
+      }
+      th[s][a * LF + b] = theta;
+    }
+    __syncthreads();
+  }
+
+  // Final fluxes, SSP-RK3 combine (rk3_kernel SC:1592-1611), repair, clamp.
+  const int a = LH + tid / LT, b = LH + tid % LT;
+  const int i = i0 - LH + a, j = j0 - LH + b;
+  double local_dt = 1.0e30;
+  if (interior(i, j)) {
+    const size_t c = (size_t)i * ld + (size_t)j;
+    CellFaces cf;
+    faces_of(a, b, 3, cf);
+    const double ap = U.f[5][c];
+    double un[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      double rh, rl;
+      residual(F, m, c, ld, cf, 3, inv_dx, inv_dy, ap, rh, rl);
+      un[m] = alpha * Un.f[m][c] + beta * (U.f[m][c] - dt * rh);
+    }
+    if (repair) repair_cell(e, un);
+    un[5] = clamp_al(un[5]);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) Uo.f[m][c] = un[m];
+    if (dt_acc) local_dt = cell_dt(e, un, G.dx, G.dy);
+  }
+  if (dt_acc) block_min_atomic(local_dt, dt_acc);
+}
+
+// ========================================================================
+// small kernels
+// ========================================================================
+constexpr int STHREADS = 256;
+
+__global__ void __launch_bounds__(STHREADS) k_dt(Geo G, Eos e, State U, double* dt_acc) {
+  const int n = G.nx * G.ny;
+  const int k = blockIdx.x * STHREADS + threadIdx.x;
+  double v = 1.0e30;
+  if (k < n) {
+    const size_t c = (size_t)(G.g + k / G.ny) * (size_t)G.ld + (size_t)(G.g + k % G.ny);
+    double u[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = U.f[m][c];
+    v = cell_dt(e, u, G.dx, G.dy);
+  }
+  // same block reduction as the fused epilogue
+  __shared__ double wmins[STHREADS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = wmin(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) wmins[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmins[0];
+    for (int w = 1; w < STHREADS / 64; ++w) m = wmin(m, wmins[w]);
+    atomicMin((unsigned long long*)dt_acc, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// dt = cfl * min (get_dt SC:1995); reset the accumulator to 1e30 (SC:1987).
+__global__ void k_dt_finalize(double* dt_acc, double* dt_out, double cfl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double m = *dt_acc;
+    *dt_out = cfl * m;
+    *dt_acc = 1.0e30;
+  }
+}
+
+__global__ void k_fill(double* p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Interior repair + clamp (the initial sequence SC:1998-2001).
+__global__ void __launch_bounds__(STHREADS) k_prepare(Geo G, Eos e, State U, int repair) {
+  const int n = G.nx * G.ny;
+  const int k = blockIdx.x * STHREADS + threadIdx.x;
+  if (k >= n) return;
+  const size_t c = (size_t)(G.g + k / G.ny) * (size_t)G.ld + (size_t)(G.g + k % G.ny);
+  double u[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) u[m] = U.f[m][c];
+  if (repair) repair_cell(e, u);
+  u[5] = clamp_al(u[5]);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) U.f[m][c] = u[m];
+}
+
+// enforce_admissible_kernel over the padded grid (SC:918-958).
+__global__ void __launch_bounds__(STHREADS) k_enforce(Geo G, Eos e, State U) {
+  const int nyg = G.ny + 2 * G.g, nxg = G.nx + 2 * G.g;
+  const int k = blockIdx.x * STHREADS + threadIdx.x;
+  if (k >= nxg * nyg) return;
+  const size_t c = (size_t)(k / nyg) * (size_t)G.ld + (size_t)(k % nyg);
+  double u[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) u[m] = U.f[m][c];
+  repair_cell(e, u);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) U.f[m][c] = u[m];
+}
+
+// cons_to_prim_kernel over the padded grid (SC:840-864).
+__global__ void __launch_bounds__(STHREADS) k_cons_to_prim(Geo G, Eos e, State U, double* up,
+                                                           double* vp, double* pp) {
+  const int nyg = G.ny + 2 * G.g, nxg = G.nx + 2 * G.g;
+  const int k = blockIdx.x * STHREADS + threadIdx.x;
+  if (k >= nxg * nyg) return;
+  const size_t c = (size_t)(k / nyg) * (size_t)G.ld + (size_t)(k % nyg);
+  double a1 = clamp_al(U.f[5][c]);
+  U.f[5][c] = a1;
+  double uu, vv;
+  double p = pressure(e, U.f[0][c], U.f[1][c], U.f[2][c], U.f[3][c], U.f[4][c], a1, uu, vv);
+  up[c] = uu;
+  vp[c] = vv;
+  pp[c] = p;
+}
+
+// bc_x then bc_y over all ghost cells (SC:1616-1682); each ghost cell reads
+// its mapped interior cell, which no thread writes.
+__global__ void __launch_bounds__(STHREADS) k_bc(Geo G, State U, double* up, double* vp,
+                                                 double* pp) {
+  const int nyg = G.ny + 2 * G.g, nxg = G.nx + 2 * G.g;
+  const int k = blockIdx.x * STHREADS + threadIdx.x;
+  if (k >= nxg * nyg) return;
+  const int i = k / nyg, j = k % nyg;
+  if (i >= G.g && i < G.g + G.nx && j >= G.g && j < G.g + G.ny) return;
+  double sru, srv;
+  const size_t src = map_cell(G, i, j, sru, srv);
+  const size_t c = (size_t)i * (size_t)G.ld + (size_t)j;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    double v = U.f[m][src];
+    if (m == 2 && sru < 0.0) v = -1.0 * v;
+    if (m == 3 && srv < 0.0) v = -1.0 * v;
+    U.f[m][c] = v;
+  }
+  if (up) {
+    double uu = up[src], vv = vp[src];
+    if (sru < 0.0) uu = -1.0 * uu;
+    if (srv < 0.0) vv = -1.0 * vv;
+    up[c] = uu;
+    vp[c] = vv;
+    pp[c] = pp[src];
+  }
+}
+
+// ========================================================================
+// launch helpers (called by the C-ABI layer)
+// ========================================================================
+typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
+
+template <int BASE, int SHEAR, int FACEPP>
+static void launch_faces_t(const Geo& G, const Eos& e, const State& U, const Faces& F,
+                           int vf_always, double t1, hipStream_t st) {
+  dim3 grid((G.ny + 1 + FTJ - 1) / FTJ, (G.nx + 1 + FTI - 1) / FTI);
+  hipLaunchKernelGGL((k_faces<BASE, SHEAR, FACEPP>), grid, dim3(FTHREADS), 0, st, G, e, U, F,
+                     vf_always, t1);
+}
+
+}  // namespace wamp
+
+// Exposed to wamp_capi.cpp
+extern "C" {
+using namespace wamp;
+
+hipError_t wamp_launch_faces(const Geo* G, const Eos* e, const State* U, const Faces* F,
+                             int base, int shear, int face_pp, int vf_always, double t1,
+                             hipStream_t st) {
+#define WAMP_F(B, S, P)                                                  \
+  if (base == B && shear == S && face_pp == P) {                        \
+    launch_faces_t<B, S, P>(*G, *e, *U, *F, vf_always, t1, st);         \
+    return hipGetLastError();                                           \
+  }
+  WAMP_F(0, 0, 0) WAMP_F(0, 0, 1) WAMP_F(0, 1, 0) WAMP_F(0, 1, 1) WAMP_F(0, 2, 0) WAMP_F(0, 2, 1)
+  WAMP_F(1, 0, 0) WAMP_F(1, 0, 1) WAMP_F(1, 1, 0) WAMP_F(1, 1, 1) WAMP_F(1, 2, 0) WAMP_F(1, 2, 1)
+#undef WAMP_F
+  return hipErrorInvalidValue;
+}
+
+hipError_t wamp_launch_limit_update(const Geo* G, const Eos* e, const State* Un, const State* U,
+                                    const Faces* F, const State* Uo, double alpha, double beta,
+                                    const double* dt_dev, double* dt_acc, int repair,
+                                    hipStream_t st) {
+  dim3 grid((G->ny + LT - 1) / LT, (G->nx + LT - 1) / LT);
+  hipLaunchKernelGGL(k_limit_update, grid, dim3(LTHREADS), 0, st, *G, *e, *Un, *U, *F, *Uo, alpha,
+                     beta, dt_dev, dt_acc, repair);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_dt(const Geo* G, const Eos* e, const State* U, double* dt_acc,
+                          double* dt_out, double cfl, hipStream_t st) {
+  const int n = G->nx * G->ny;
+  hipLaunchKernelGGL(k_dt, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G, *e,
+                     *U, dt_acc);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, dt_acc, dt_out, cfl);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t st) {
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, dt_acc, dt_out, cfl);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_fill(double* p, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, p, v);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_prepare(const Geo* G, const Eos* e, const State* U, int repair,
+                               hipStream_t st) {
+  const int n = G->nx * G->ny;
+  hipLaunchKernelGGL(k_prepare, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G,
+                     *e, *U, repair);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_enforce(const Geo* G, const Eos* e, const State* U, hipStream_t st) {
+  const int n = (G->nx + 2 * G->g) * (G->ny + 2 * G->g);
+  hipLaunchKernelGGL(k_enforce, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G,
+                     *e, *U);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_cons_to_prim(const Geo* G, const Eos* e, const State* U, double* up,
+                                    double* vp, double* pp, hipStream_t st) {
+  const int n = (G->nx + 2 * G->g) * (G->ny + 2 * G->g);
+  hipLaunchKernelGGL(k_cons_to_prim, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st,
+                     *G, *e, *U, up, vp, pp);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_bc(const Geo* G, const State* U, double* up, double* vp, double* pp,
+                          hipStream_t st) {
+  const int n = (G->nx + 2 * G->g) * (G->ny + 2 * G->g);
+  hipLaunchKernelGGL(k_bc, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G, *U, up,
+                     vp, pp);
+  return hipGetLastError();
+}
+}
