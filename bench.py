@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 SSP-RK3 step on one MI355X: Gcell-updates/s.
+
+Workload: the Mach-10 water-cylinder case, 4096x2048 cells on [0,4]x[0,2]
+(water cylinder r=0.25 at (1,1), gamma=4.4, p_inf=6000 in water, air
+gamma=1.4; Mach 10 shock at x=0.6), the M10 scheme (MUSCL + THINC, Wong face
+and flux limiters), CFL 0.4, FP64.  Synthetic initial state built on the
+host, resident in HBM before the timed region.
+
+One step = one full SSP-RK3 step (3 stages of fluxes, 3-stage Wong limiter,
+RK combine, repair, alpha clamp) plus the step's dt reduction.
+value = NX*NY*steps / t / 1e9.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Also reported, for the dominant kernel (the face-flux kernel), its average
+launch duration from hipEvents recorded around it inside the timed region,
+and as cpu_baseline the CPU oracle on the host cores over a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "wave-multiphase_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+ALG_BYTES_PER_CELL_UPDATE = 384.0  # FP64 state: 96 + 144 + 144 B over the 3 stages
+
+
+def cpu_threads() -> int:
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(cfg, U0, steps: int):
+    """CPU oracle (test infrastructure) on the same workload: `steps` steps, all threads."""
+    from oracle.oracle import Oracle
+
+    th = cpu_threads()
+    orc = Oracle(cfg, threads=th)
+    orc.set_state(U0)
+    orc.prepare()
+    dt = orc.get_dt()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.step(dt)
+        dt = orc.get_dt()
+    el = time.perf_counter() - t0
+    orc.close()
+    g = cfg.grid
+    return {
+        "value": g.nx * g.ny * steps / el / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": th,
+        "kind": "port",
+        "sample": f"{steps} SSP-RK3 step(s) of the same {g.nx}x{g.ny} case, C restatement "
+                  f"of the reference kernels (OpenMP, {th} threads), {el:.1f} s",
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one case per GPU; "
+              "a single case would slab-decompose with halo exchange, not implemented). "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from wamp import cases
+    from wamp.config import SCHEME_M10
+    from wamp import _lib
+    from wamp.solver import Solver
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 2
+    cfg, U0 = cases.mach10_cylinder(args.nx, args.ny, scheme=SCHEME_M10)
+    g = cfg.grid
+    s = Solver(cfg, device="cuda:0")
+    s.set_state(U0)
+    s.prepare()
+    s.compute_dt()
+    for _ in range(args.warmup):
+        s.step_device()
+    torch.cuda.synchronize()
+
+    K = args.steps
+    _lib.check(s.L.wamp_timing_enable(s.ctx, K), "wamp_timing_enable")
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(K):
+        s.step_device()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_ms = e0.elapsed_time(e1)
+    import ctypes as C
+
+    ms = (C.c_double * 2)()
+    nl = (C.c_int64 * 2)()
+    _lib.check(s.L.wamp_timing_read(s.ctx, ms, nl), "wamp_timing_read")
+    _lib.check(s.L.wamp_timing_enable(s.ctx, 0), "wamp_timing_enable")
+    dt_now = float(s.dt_buf[0].item())
+    finite = bool(torch.isfinite(s.U).all().item())
+    cells = g.nx * g.ny
+    value = cells * K / (t_ms / 1e3) / 1e9
+    faces_ms = ms[0] / max(nl[0], 1)
+    lu_ms = ms[1] / max(nl[1], 1)
+    # algorithmic bytes of one face-kernel launch: one stage's share of the
+    # step's 384 B per cell-update, over all NX*NY cells
+    alg_bytes = ALG_BYTES_PER_CELL_UPDATE / 3.0 * cells
+    achieved = alg_bytes / (faces_ms / 1e3) / 1e9
+    step_gbs = ALG_BYTES_PER_CELL_UPDATE * cells * K / (t_ms / 1e3) / 1e9
+
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(value, 6),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_ms / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mach10_cylinder_{g.nx}x{g.ny}",
+            "grid": [g.nx, g.ny],
+            "scheme": "M10 (MUSCL + THINC, Wong face+flux limiters)",
+            "precision": "fp64-plain",
+            "cfl": cfg.scheme.cfl,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_faces",
+            "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "launch_ms": round(faces_ms, 4),
+            "launches": int(nl[0]),
+        },
+        "kernels_ms": {"k_faces": round(faces_ms, 4), "k_limit_update": round(lu_ms, 4)},
+        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),
+        "state_finite": finite,
+        "dt_last": dt_now,
+    }
+    if args.cpu_steps > 0:
+        out["cpu_baseline"] = cpu_baseline(cfg, U0, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    s.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

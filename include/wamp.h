@@ -120,6 +120,15 @@ int wamp_get_dt(wamp_ctx *ctx, double *const U[6], double *dt_dev, void *stream)
 int wamp_step(wamp_ctx *ctx, double *const U[6], const double *dt_dev, double *dt_next_dev,
               void *stream);
 
+/* Measurement hook (no counterpart in the reference; for benchmarks and
+   profilers).  wamp_timing_enable(ctx, n) makes the next n calls of
+   wamp_step record hipEvents on their stream around each stage's face-flux
+   kernel and limiter/update kernel (n = 0 disables).  wamp_timing_read
+   waits for the recorded events and returns the summed milliseconds and
+   launch counts: [0] face-flux kernel, [1] limiter/update kernel. */
+int wamp_timing_enable(wamp_ctx *ctx, int32_t max_steps);
+int wamp_timing_read(wamp_ctx *ctx, double ms[2], int64_t launches[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,7 +63,13 @@ struct wamp_ctx {
   Faces faces;
   State u1, u2;
   double* dt_acc;
+  // measurement hook: events around the kernels of up to timing_cap steps
+  int timing_cap;
+  int timing_n;
+  hipEvent_t* ev;  // 7 per step: faces/limit_update boundaries of 3 stages
 };
+
+static constexpr int EV_PER_STEP = 7;
 
 static thread_local std::string g_err;
 
@@ -160,8 +166,19 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   return 0;
 }
 
+static void timing_free(wamp_ctx* c) {
+  if (c->ev) {
+    for (int k = 0; k < c->timing_cap * EV_PER_STEP; ++k) (void)hipEventDestroy(c->ev[k]);
+    delete[] c->ev;
+  }
+  c->ev = nullptr;
+  c->timing_cap = 0;
+  c->timing_n = 0;
+}
+
 int wamp_ctx_destroy(wamp_ctx* c) {
   if (!c) return 0;
+  timing_free(c);
   int rc = 0;
   if (c->block) rc = hip_check(hipFree(c->block), "hipFree(scratch)");
   delete c;
@@ -265,24 +282,72 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
   if (!dt_dev || !dt_next_dev) return fail(-2, "dt_dev and dt_next_dev must be non-NULL");
   hipStream_t st = (hipStream_t)stream;
   const int rep = c->scheme.repair;
-  // stage 1: U1 = Un + dt R(Un)            (SC:2031-2037)
-  if (int rc = launch_faces(c, a, c->faces, st)) return rc;
-  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &a, &c->faces, &c->u1, 0.0,
-                                                  1.0, dt_dev, nullptr, rep, st), "stage 1"))
-    return rc;
-  // stage 2: U2 = 3/4 Un + 1/4 (U1 + dt R(U1))   (SC:2040-2046)
-  if (int rc = launch_faces(c, c->u1, c->faces, st)) return rc;
-  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &c->u1, &c->faces, &c->u2,
-                                                  0.75, 0.25, dt_dev, nullptr, rep, st), "stage 2"))
-    return rc;
+  hipEvent_t* ev = nullptr;
+  if (c->timing_n < c->timing_cap) ev = c->ev + (size_t)c->timing_n++ * EV_PER_STEP;
+  auto mark = [&](int k) -> int {
+    return ev ? hip_check(hipEventRecord(ev[k], st), "hipEventRecord") : 0;
+  };
+  const State* cur[3] = {&a, &c->u1, &c->u2};
+  const State* dst[3] = {&c->u1, &c->u2, &a};
+  const double coef[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
+  // stage 1: U1 = Un + dt R(Un)                               (SC:2031-2037)
+  // stage 2: U2 = 3/4 Un + 1/4 (U1 + dt R(U1))                (SC:2040-2046)
   // stage 3: Un+1 = 1/3 Un + 2/3 (U2 + dt R(U2)), written over Un, with the
-  // CFL minimum of the new state folded in   (SC:2049-2059)
-  if (int rc = launch_faces(c, c->u2, c->faces, st)) return rc;
-  if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, &c->u2, &c->faces, &a,
-                                                  1.0 / 3.0, 2.0 / 3.0, dt_dev, c->dt_acc, rep, st),
-                         "stage 3"))
-    return rc;
+  //          CFL minimum of the new state folded in           (SC:2049-2059)
+  if (int rc = mark(0)) return rc;
+  for (int s = 0; s < 3; ++s) {
+    if (int rc = launch_faces(c, *cur[s], c->faces, st)) return rc;
+    if (int rc = mark(2 * s + 1)) return rc;
+    if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, cur[s], &c->faces,
+                                                    dst[s], coef[s][0], coef[s][1], dt_dev,
+                                                    s == 2 ? c->dt_acc : nullptr, rep, st),
+                           "limit_update"))
+      return rc;
+    if (int rc = mark(2 * s + 2)) return rc;
+  }
   return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, c->scheme.cfl, st), "dt");
+}
+
+int wamp_timing_enable(wamp_ctx* c, int32_t max_steps) {
+  if (int rc = check_ctx(c)) return rc;
+  if (max_steps < 0 || max_steps > 100000) return fail(-3, "max_steps must be in [0, 100000]");
+  timing_free(c);
+  if (max_steps == 0) return 0;
+  c->ev = new (std::nothrow) hipEvent_t[(size_t)max_steps * EV_PER_STEP];
+  if (!c->ev) return fail(-5, "host allocation failed");
+  for (int k = 0; k < max_steps * EV_PER_STEP; ++k) {
+    if (int rc = hip_check(hipEventCreate(&c->ev[k]), "hipEventCreate")) {
+      for (int q = 0; q < k; ++q) (void)hipEventDestroy(c->ev[q]);
+      delete[] c->ev;
+      c->ev = nullptr;
+      return rc;
+    }
+  }
+  c->timing_cap = max_steps;
+  c->timing_n = 0;
+  return 0;
+}
+
+int wamp_timing_read(wamp_ctx* c, double ms[2], int64_t launches[2]) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!ms || !launches) return fail(-2, "ms and launches must be non-NULL");
+  ms[0] = ms[1] = 0.0;
+  launches[0] = launches[1] = 0;
+  for (int n = 0; n < c->timing_n; ++n) {
+    hipEvent_t* ev = c->ev + (size_t)n * EV_PER_STEP;
+    if (int rc = hip_check(hipEventSynchronize(ev[EV_PER_STEP - 1]), "hipEventSynchronize")) return rc;
+    for (int s = 0; s < 3; ++s) {
+      float a = 0.f, b = 0.f;
+      if (int rc = hip_check(hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]), "elapsed")) return rc;
+      if (int rc = hip_check(hipEventElapsedTime(&b, ev[2 * s + 1], ev[2 * s + 2]), "elapsed")) return rc;
+      ms[0] += a;
+      ms[1] += b;
+      launches[0] += 1;
+      launches[1] += 1;
+    }
+  }
+  c->timing_n = 0;
+  return 0;
 }
 
 }  // extern "C"

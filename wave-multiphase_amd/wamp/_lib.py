@@ -54,6 +54,8 @@ SIGNATURES = [
                                     C.c_double, _VP, _VP, _VP]),
     ("wamp_get_dt", C.c_int, [_VP, PtrArr6, _VP, _VP]),
     ("wamp_step", C.c_int, [_VP, PtrArr6, _VP, _VP, _VP]),
+    ("wamp_timing_enable", C.c_int, [_VP, C.c_int32]),
+    ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
 ]
 
 _lib = None

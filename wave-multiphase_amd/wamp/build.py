@@ -1,6 +1,6 @@
 """Compile libwamp.so in-tree with hipcc for gfx950.
 
-    python -m wamp.build            (with wave-multiphase_amd/ on sys.path)
+    python wave-multiphase_amd/wamp/build.py
 """
 from __future__ import annotations
 
@@ -8,7 +8,11 @@ import os
 import subprocess
 import sys
 
-from ._lib import CSRC_DIR, LIB_PATH
+try:
+    from ._lib import CSRC_DIR, LIB_PATH
+except ImportError:  # run as a script
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from wamp._lib import CSRC_DIR, LIB_PATH
 
 REPO = os.path.dirname(os.path.dirname(CSRC_DIR))
 INCLUDE = os.path.join(REPO, "include")
