@@ -78,6 +78,9 @@ def main(argv=None):
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
+                         "of known size, to scale hardware traffic counters)")
     args = ap.parse_args(argv)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,6 +100,13 @@ def main(argv=None):
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     cfg, U0 = cases.mach10_cylinder(args.nx, args.ny, scheme=SCHEME_M10)
     g = cfg.grid
     s = Solver(cfg, device="cuda:0")
