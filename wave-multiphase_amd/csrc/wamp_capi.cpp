@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -40,6 +41,10 @@ hipError_t wamp_launch_limit_update(const Geo*, const Eos*, const State* Un, con
                                     const Faces*, const State* Uo, double alpha, double beta,
                                     const double* dt_dev, double* dt_acc, int repair,
                                     hipStream_t);
+hipError_t wamp_launch_lu_march(const Geo*, const Eos*, const State* Un, const State* U,
+                               const Faces*, const State* Uo, double alpha, double beta,
+                               const double* dt_dev, double* dt_acc, int repair, int seg,
+                               hipStream_t);
 hipError_t wamp_launch_dt(const Geo*, const Eos*, const State*, double* dt_acc, double* dt_out,
                           double cfl, hipStream_t);
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t);
@@ -58,6 +63,8 @@ struct wamp_ctx {
   int precision;
   int device;
   double t1;          // tanh(beta_thinc / 2), SC:334
+  int lu_kind;        // 0: LDS-tile limiter/update kernel, 1: register-marching kernel
+  int lu_seg;         // rows per wave of the marching kernel
   double* block;      // one allocation: faces | stage buffers | dt accumulator
   int64_t bytes;
   Faces faces;
@@ -141,6 +148,10 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   c->scheme = *scheme;
   c->precision = precision;
   c->t1 = std::tanh(1.8 * 0.5);
+  c->lu_kind = 1;
+  c->lu_seg = 16;
+  if (const char* v = std::getenv("WAMP_LU_KIND")) c->lu_kind = std::atoi(v);
+  if (const char* v = std::getenv("WAMP_LU_SEG")) c->lu_seg = std::atoi(v);
   int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
   if (rc) { delete c; return rc; }
 
@@ -225,6 +236,17 @@ int wamp_prepare(wamp_ctx* c, double* const U[6], void* stream) {
                    "prepare");
 }
 
+static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& f,
+                     const State& uo, double alpha, double beta, const double* dt_dev,
+                     double* dt_acc, hipStream_t st) {
+  hipError_t e = c->lu_kind == 1
+                     ? wamp_launch_lu_march(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev,
+                                            dt_acc, c->scheme.repair, c->lu_seg, st)
+                     : wamp_launch_limit_update(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta,
+                                                dt_dev, dt_acc, c->scheme.repair, st);
+  return hip_check(e, "limit_update");
+}
+
 static int launch_faces(wamp_ctx* c, const State& s, const Faces& f, hipStream_t st) {
   return hip_check(wamp_launch_faces(&c->geo, &c->eos, &s, &f, c->scheme.base, c->scheme.shear,
                                      c->scheme.face_pp, c->scheme.vf_thinc, c->t1, st),
@@ -260,9 +282,7 @@ int wamp_limit_update(wamp_ctx* c, double* const Un[6], double* const U[6], doub
     if (!F[m]) return fail(-2, "F[%d] is NULL", m);
     f.f[m] = F[m];
   }
-  return hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &sn, &s, &f, &so, alpha, beta, dt_dev,
-                                            dt_acc, c->scheme.repair, (hipStream_t)stream),
-                   "limit_update");
+  return launch_lu(c, sn, s, f, so, alpha, beta, dt_dev, dt_acc, (hipStream_t)stream);
 }
 
 int wamp_get_dt(wamp_ctx* c, double* const U[6], double* dt_dev, void* stream) {
@@ -281,7 +301,6 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
   if (int rc = state_of(U, a, "U")) return rc;
   if (!dt_dev || !dt_next_dev) return fail(-2, "dt_dev and dt_next_dev must be non-NULL");
   hipStream_t st = (hipStream_t)stream;
-  const int rep = c->scheme.repair;
   hipEvent_t* ev = nullptr;
   if (c->timing_n < c->timing_cap) ev = c->ev + (size_t)c->timing_n++ * EV_PER_STEP;
   auto mark = [&](int k) -> int {
@@ -298,10 +317,8 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
   for (int s = 0; s < 3; ++s) {
     if (int rc = launch_faces(c, *cur[s], c->faces, st)) return rc;
     if (int rc = mark(2 * s + 1)) return rc;
-    if (int rc = hip_check(wamp_launch_limit_update(&c->geo, &c->eos, &a, cur[s], &c->faces,
-                                                    dst[s], coef[s][0], coef[s][1], dt_dev,
-                                                    s == 2 ? c->dt_acc : nullptr, rep, st),
-                           "limit_update"))
+    if (int rc = launch_lu(c, a, *cur[s], c->faces, *dst[s], coef[s][0], coef[s][1], dt_dev,
+                           s == 2 ? c->dt_acc : nullptr, st))
       return rc;
     if (int rc = mark(2 * s + 2)) return rc;
   }

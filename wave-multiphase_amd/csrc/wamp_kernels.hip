@@ -536,6 +536,284 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
 }
 
 // ========================================================================
+// k_lu_march -- the limiter + update of k_limit_update as a register
+// pipeline.  One wave owns 64 consecutive j (58 output columns plus a 3-lane
+// halo each side: the update of a cell needs theta2 at +-1, theta1 at +-2,
+// theta0 at +-3 cells) and marches along i.  At front row f it forms
+// theta0(f), theta1(f-1), theta2(f-2) and the update of row f-3.  Every
+// face value is fetched from HBM once per use-group and kept in registers
+// while it is live:
+//   x-faces  components {0,1} of row f, {5,6} of row f-1 and all of row f-2
+//            are loaded at front f; rows f-4..f-2 (all components) stay in
+//            registers for theta2 and the update;
+//   y-faces  the face above the cell is loaded; the face below is the
+//            neighbouring lane's (lane shuffle), except for theta0, whose
+//            lower faces are loaded so that theta0 is exact on every lane;
+//            row f-2's values stay in registers for the update one front later.
+// ========================================================================
+constexpr int MW = 64;
+constexpr int MH = 3;
+constexpr int MJ = MW - 2 * MH;
+
+__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, MW); }
+__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, MW); }
+
+// f <- L + theta_k (f - L) for k < nb (SC:1506-1512 once per limiter stage)
+__device__ __forceinline__ double blend3(double H, double L, const double* th, int nb) {
+  double f = H;
+  for (int k = 0; k < nb; ++k) f = L + th[k] * (f - L);
+  return f;
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(256, 2)
+k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
+           const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+  const int lane = threadIdx.x & (MW - 1);
+  const int wave = threadIdx.x / MW;
+  const int j = G.g + (int)blockIdx.x * MJ - MH + lane;
+  const int i0 = G.g + ((int)blockIdx.y * (blockDim.x / MW) + wave) * SEG;
+  if (i0 >= G.g + G.nx) return;  // wave-uniform
+  const int i_end = min(i0 + SEG, G.g + G.nx);
+  const size_t ld = (size_t)G.ld;
+  const double dt = *dt_dev;
+  const double inv_dx = 1.0 / G.dx;
+  const double inv_dy = 1.0 / G.dy;
+  const bool col_in = j >= G.g && j < G.g + G.ny;          // interior column
+  const bool yface_in = j >= G.g - 1 && j < G.g + G.ny;   // y-face (r, j+1/2) exists
+  const bool out_lane = lane >= MH && lane < MH + MJ && col_in;
+  auto row_in = [&](int r) { return r >= G.g && r < G.g + G.nx; };
+  auto xrow_in = [&](int r) { return r >= G.g - 1 && r < G.g + G.nx; };  // x-face r+1/2 exists
+  auto at = [&](int r) { return (size_t)r * ld + (size_t)j; };
+  // face component m: high F.f[base + m], low F.f[base + 14 + m]; base 0 = x, 7 = y
+
+  double x01[2][4] = {};   // x-face rows f-1, f: H0 H1 L0 L1
+  double x56[2][4] = {};   // x-face rows f-2, f-1: H5 H6 L5 L6
+  double xa[2][14] = {};   // x-face rows f-3, f-2: H0..6 L0..6
+  double F2xm[7] = {};     // final flux at x-face row f-4 (below the update row)
+  double F2yu[7] = {};     // final flux at the y-faces above row f-3
+  double t0[5] = {1.0, 1.0, 1.0, 1.0, 1.0};  // theta0 rows f-4 .. f
+  double t1[4] = {1.0, 1.0, 1.0, 1.0};       // theta1 rows f-4 .. f-1
+  double t2[3] = {1.0, 1.0, 1.0};            // theta2 rows f-4 .. f-2
+  double local_dt = 1.0e30;
+
+  for (int f = i0 - 3; f < i_end + 3; ++f) {
+    // ---------------- x-face loads of this front ----------------
+    {
+      const bool ok = xrow_in(f) && col_in;
+      const size_t c = at(f);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x01[0][q] = x01[1][q];
+      x01[1][0] = ok ? F.f[0][c] : 0.0;
+      x01[1][1] = ok ? F.f[1][c] : 0.0;
+      x01[1][2] = ok ? F.f[14][c] : 0.0;
+      x01[1][3] = ok ? F.f[15][c] : 0.0;
+    }
+    {
+      const bool ok = xrow_in(f - 1) && col_in;
+      const size_t c = at(f - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x56[0][q] = x56[1][q];
+      x56[1][0] = ok ? F.f[5][c] : 0.0;
+      x56[1][1] = ok ? F.f[6][c] : 0.0;
+      x56[1][2] = ok ? F.f[19][c] : 0.0;
+      x56[1][3] = ok ? F.f[20][c] : 0.0;
+    }
+    {
+      const bool ok = xrow_in(f - 2) && col_in;
+      const size_t c = at(f - 2);
+#pragma unroll
+      for (int m = 0; m < 14; ++m) xa[0][m] = xa[1][m];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        xa[1][m] = ok ? F.f[m][c] : 0.0;
+        xa[1][7 + m] = ok ? F.f[14 + m][c] : 0.0;
+      }
+    }
+
+    // ---------------- theta0 at row f (raw high-order fluxes) ----------------
+    double th = 1.0;
+    {
+      const size_t c = at(f);
+      const bool ok = row_in(f) && col_in;
+      if (ok) {
+        double l[6], h[6];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
+          double yuh = F.f[7 + m][c], ydh = F.f[7 + m][c - 1];
+          double yul = F.f[21 + m][c], ydl = F.f[21 + m][c - 1];
+          double rh = (x01[1][m] - x01[0][m]) * inv_dx + (yuh - ydh) * inv_dy;
+          double rl = (x01[1][2 + m] - x01[0][2 + m]) * inv_dx + (yul - ydl) * inv_dy;
+          double u = U.f[m][c];
+          l[m] = u - dt * rl;
+          h[m] = u - dt * rh;
+        }
+        th = theta_stage<0>(e, l, h);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t0[k] = t0[k + 1];
+    t0[4] = th;
+
+    // ---------------- theta1 at row f-1 (fluxes blended once) ----------------
+    {
+      const int r = f - 1;
+      const size_t c = at(r);
+      const bool ok = row_in(r) && col_in;
+      const bool yok = row_in(r) && yface_in;
+      double tf[4];
+      tf[0] = face_theta(t0[2], t0[3]);
+      tf[1] = face_theta(t0[3], t0[4]);
+      tf[2] = face_theta(lane_up(t0[3]), t0[3]);
+      tf[3] = face_theta(t0[3], lane_down(t0[3]));
+      double yu[4];
+      yu[0] = yok ? F.f[12][c] : 0.0;
+      yu[1] = yok ? F.f[13][c] : 0.0;
+      yu[2] = yok ? F.f[26][c] : 0.0;
+      yu[3] = yok ? F.f[27][c] : 0.0;
+      double yd[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) yd[q] = lane_up(yu[q]);
+      double th1 = 1.0;
+      if (ok) {
+        const double al = U.f[5][c];
+        double Hxp5 = blend3(x56[1][0], x56[1][2], &tf[1], 1);
+        double Hxm5 = blend3(x56[0][0], x56[0][2], &tf[0], 1);
+        double Hyp5 = blend3(yu[0], yu[2], &tf[3], 1);
+        double Hym5 = blend3(yd[0], yd[2], &tf[2], 1);
+        double Hxp6 = blend3(x56[1][1], x56[1][3], &tf[1], 1);
+        double Hxm6 = blend3(x56[0][1], x56[0][3], &tf[0], 1);
+        double Hyp6 = blend3(yu[1], yu[3], &tf[3], 1);
+        double Hym6 = blend3(yd[1], yd[3], &tf[2], 1);
+        double rh = (Hxp5 - Hxm5) * inv_dx + (Hyp5 - Hym5) * inv_dy;
+        double rl = (x56[1][2] - x56[0][2]) * inv_dx + (yu[2] - yd[2]) * inv_dy;
+        rh = rh - al * (Hxp6 - Hxm6) * inv_dx - al * (Hyp6 - Hym6) * inv_dy;
+        rl = rl - al * (x56[1][3] - x56[0][3]) * inv_dx - al * (yu[3] - yd[3]) * inv_dy;
+        double l[6], h[6];
+        l[5] = al - dt * rl;
+        h[5] = al - dt * rh;
+        th1 = theta_stage<1>(e, l, h);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t1[k] = t1[k + 1];
+      t1[3] = th1;
+    }
+
+    // ---------------- theta2 at row f-2 (fluxes blended twice) ----------------
+    double yn[14];  // y-faces above row f-2 (H0..6, L0..6)
+    {
+      const int r = f - 2;
+      const size_t c = at(r);
+      const bool ok = row_in(r) && col_in;
+      const bool yok = row_in(r) && yface_in;
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        yn[m] = yok ? F.f[7 + m][c] : 0.0;
+        yn[7 + m] = yok ? F.f[21 + m][c] : 0.0;
+      }
+      double tf[4][2];
+      tf[0][0] = face_theta(t0[1], t0[2]);
+      tf[1][0] = face_theta(t0[2], t0[3]);
+      tf[2][0] = face_theta(lane_up(t0[2]), t0[2]);
+      tf[3][0] = face_theta(t0[2], lane_down(t0[2]));
+      tf[0][1] = face_theta(t1[1], t1[2]);
+      tf[1][1] = face_theta(t1[2], t1[3]);
+      tf[2][1] = face_theta(lane_up(t1[2]), t1[2]);
+      tf[3][1] = face_theta(t1[2], lane_down(t1[2]));
+      double ydn[14];
+#pragma unroll
+      for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
+      double th2 = 1.0;
+      if (ok) {
+        const double al = U.f[5][c];
+        // component 6 (contact velocity) first: it enters component 5's residual
+        const double s_hm = blend3(xa[0][6], xa[0][13], tf[0], 2);
+        const double s_hp = blend3(xa[1][6], xa[1][13], tf[1], 2);
+        const double s_hd = blend3(ydn[6], ydn[13], tf[2], 2);
+        const double s_hu = blend3(yn[6], yn[13], tf[3], 2);
+        double l[6], h[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double hm = blend3(xa[0][m], xa[0][7 + m], tf[0], 2);
+          double hp = blend3(xa[1][m], xa[1][7 + m], tf[1], 2);
+          double hd = blend3(ydn[m], ydn[7 + m], tf[2], 2);
+          double hu = blend3(yn[m], yn[7 + m], tf[3], 2);
+          double rh = (hp - hm) * inv_dx + (hu - hd) * inv_dy;
+          double rl = (xa[1][7 + m] - xa[0][7 + m]) * inv_dx + (yn[7 + m] - ydn[7 + m]) * inv_dy;
+          if (m == 5) {
+            rh = rh - al * (s_hp - s_hm) * inv_dx - al * (s_hu - s_hd) * inv_dy;
+            rl = rl - al * (xa[1][13] - xa[0][13]) * inv_dx - al * (yn[13] - ydn[13]) * inv_dy;
+          }
+          const double u = U.f[m][c];
+          l[m] = u - dt * rl;
+          h[m] = u - dt * rh;
+        }
+        th2 = theta_stage<2>(e, l, h);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) t2[k] = t2[k + 1];
+      t2[2] = th2;
+    }
+
+    // ---- final fluxes (three blends) of the faces whose thetas are now complete:
+    //      x-face row f-3 (between rows f-3, f-2) and the y-faces above row f-2
+    double F2xp[7], F2yn[7];
+    {
+      double tx[3], ty[3];
+      const double* tt[3] = {t0 + 1, t1 + 1, t2 + 1};  // rows f-3 at [0], f-2 at [1]
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        tx[k] = face_theta(tt[k][0], tt[k][1]);
+        ty[k] = face_theta(tt[k][1], lane_down(tt[k][1]));
+      }
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        F2xp[m] = blend3(xa[0][m], xa[0][7 + m], tx, 3);
+        F2yn[m] = blend3(yn[m], yn[7 + m], ty, 3);
+      }
+    }
+
+    // ---------------- update of row f-3 ----------------
+    {
+      const int r = f - 3;
+      const size_t c = at(r);
+      double yd[7];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) yd[m] = lane_up(F2yu[m]);
+      if (r >= i0 && r < i_end && out_lane) {
+        const double ap = U.f[5][c];
+        double un[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double res = (F2xp[m] - F2xm[m]) * inv_dx + (F2yu[m] - yd[m]) * inv_dy;
+          if (m == 5)
+            res = res - ap * (F2xp[6] - F2xm[6]) * inv_dx - ap * (F2yu[6] - yd[6]) * inv_dy;
+          un[m] = alpha * Un.f[m][c] + beta * (U.f[m][c] - dt * res);
+        }
+        if (repair) repair_cell(e, un);
+        un[5] = clamp_al(un[5]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) Uo.f[m][c] = un[m];
+        if (dt_acc) local_dt = wmin(local_dt, cell_dt(e, un, G.dx, G.dy));
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      F2xm[m] = F2xp[m];
+      F2yu[m] = F2yn[m];
+    }
+  }
+  if (dt_acc) {
+    double v = local_dt;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = wmin(v, __shfl_xor(v, off, 64));
+    if (lane == 0)
+      atomicMin((unsigned long long*)dt_acc, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// ========================================================================
 // small kernels
 // ========================================================================
 constexpr int STHREADS = 256;
@@ -692,6 +970,24 @@ hipError_t wamp_launch_limit_update(const Geo* G, const Eos* e, const State* Un,
   hipLaunchKernelGGL(k_limit_update, grid, dim3(LTHREADS), 0, st, *G, *e, *Un, *U, *F, *Uo, alpha,
                      beta, dt_dev, dt_acc, repair);
   return hipGetLastError();
+}
+
+hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, const State* U,
+                               const Faces* F, const State* Uo, double alpha, double beta,
+                               const double* dt_dev, double* dt_acc, int repair, int seg,
+                               hipStream_t st) {
+  const int waves_per_block = 4;
+  const int nseg = (G->nx + seg - 1) / seg;
+  dim3 grid((G->ny + MJ - 1) / MJ, (nseg + waves_per_block - 1) / waves_per_block);
+#define WAMP_SEG(S)                                                                          \
+  if (seg == S) {                                                                            \
+    hipLaunchKernelGGL(k_lu_march<S>, grid, dim3(64 * waves_per_block), 0, st, *G, *e, *Un, \
+                       *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);                    \
+    return hipGetLastError();                                                                \
+  }
+  WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(32) WAMP_SEG(64)
+#undef WAMP_SEG
+  return hipErrorInvalidValue;
 }
 
 hipError_t wamp_launch_dt(const Geo* G, const Eos* e, const State* U, double* dt_acc,
