@@ -69,6 +69,48 @@ def cpu_baseline(cfg, U0, steps: int):
     }
 
 
+def mixed_leg(cfg, U0, warmup: int, steps: int):
+    """Same case and step count in the mixed precision mode (FP32 face arithmetic)."""
+    import dataclasses
+    import ctypes as C
+
+    import torch
+
+    from wamp import _lib
+    from wamp.config import PREC_MIXED
+    from wamp.solver import Solver
+
+    g = cfg.grid
+    s = Solver(dataclasses.replace(cfg, precision=PREC_MIXED), device="cuda:0")
+    s.set_state(U0)
+    s.prepare()
+    s.compute_dt()
+    for _ in range(warmup):
+        s.step_device()
+    _lib.check(s.L.wamp_timing_enable(s.ctx, steps), "wamp_timing_enable")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        s.step_device()
+    e1.record()
+    torch.cuda.synchronize()
+    t_ms = e0.elapsed_time(e1)
+    ms = (C.c_double * 2)()
+    nl = (C.c_int64 * 2)()
+    _lib.check(s.L.wamp_timing_read(s.ctx, ms, nl), "wamp_timing_read")
+    s.close()
+    return {
+        "value": round(g.nx * g.ny * steps / (t_ms / 1e3) / 1e9, 6),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(t_ms / steps, 4),
+        "precision": "fp32 faces (energy flux fp64) + fp64 pressure/limiter/RK",
+        "kernels_ms": {"k_faces": round(ms[0] / max(nl[0], 1), 4),
+                       "k_limit_update": round(ms[1] / max(nl[1], 1), 4)},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +120,8 @@ def main(argv=None):
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
+    ap.add_argument("--mixed-steps", type=int, default=50,
+                    help="steps of the FP32-mixed leg on the same case (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
                          "of known size, to scale hardware traffic counters)")
@@ -183,6 +227,8 @@ def main(argv=None):
         "state_finite": finite,
         "dt_last": dt_now,
     }
+    if args.mixed_steps > 0:
+        out["mixed"] = mixed_leg(cfg, U0, args.warmup, args.mixed_steps)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, U0, args.cpu_steps)
     else:

@@ -36,7 +36,8 @@ using namespace wamp;
 
 extern "C" {
 hipError_t wamp_launch_faces(const Geo*, const Eos*, const State*, const Faces*, int base,
-                             int shear, int face_pp, int vf_always, double t1, hipStream_t);
+                             int shear, int face_pp, int vf_always, double t1, int precision,
+                             hipStream_t);
 hipError_t wamp_launch_limit_update(const Geo*, const Eos*, const State* Un, const State* U,
                                     const Faces*, const State* Uo, double alpha, double beta,
                                     const double* dt_dev, double* dt_acc, int repair,
@@ -133,7 +134,8 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   if (scheme->vf_thinc < 0 || scheme->vf_thinc > 1) return fail(-3, "scheme.vf_thinc must be 0 or 1");
   if (scheme->face_pp < 0 || scheme->face_pp > 1) return fail(-3, "scheme.face_pp must be 0 or 1");
   if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
-  if (precision != WAMP_PREC_FP64) return fail(-4, "precision %d is not available", precision);
+  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
+    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
 
   wamp_ctx* c = new (std::nothrow) wamp_ctx();
   if (!c) return fail(-5, "host allocation failed");
@@ -249,7 +251,7 @@ static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& 
 
 static int launch_faces(wamp_ctx* c, const State& s, const Faces& f, hipStream_t st) {
   return hip_check(wamp_launch_faces(&c->geo, &c->eos, &s, &f, c->scheme.base, c->scheme.shear,
-                                     c->scheme.face_pp, c->scheme.vf_thinc, c->t1, st),
+                                     c->scheme.face_pp, c->scheme.vf_thinc, c->t1, c->precision, st),
                    "faces");
 }
 

@@ -31,6 +31,23 @@ template <typename T> __device__ __forceinline__ T wmax(T a, T b) { return a > b
 template <typename T> __device__ __forceinline__ T wsign(T x) { return x < T(0) ? T(-1) : T(1); }
 template <typename T> __device__ __forceinline__ bool isnan_(T x) { return __builtin_isnan(x); }
 
+// Division, square root and tanh by arithmetic type.  FP64 keeps IEEE
+// correctly rounded operations (the parity path).  The FP32 face arithmetic
+// of the mixed-precision mode uses the hardware approximations (v_rcp_f32,
+// v_sqrt_f32 and an exp-based tanh, about 1-2 ulp): on CDNA4 non-packed
+// FP32 issues at the FP64 rate, so the mixed mode gains only by dropping the
+// multi-instruction IEEE sequences.
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double th(double x) { return tanh(x); }
+__device__ __forceinline__ float th(float x) {
+  float t = __expf(-2.0f * fabsf(x));
+  float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
+  return x < 0.0f ? -r : r;
+}
+
 // Reference constants, SC:44-70.
 constexpr double TINY = 1.0e-20;
 constexpr double EPSM = 1.0e-20;
@@ -54,25 +71,25 @@ constexpr double AL_CLIP = 1.0e-8;  // cons_to_prim clamp, SC:852-853
 template <typename T>
 __device__ __forceinline__ void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
   T a2 = T(1) - a1;
-  gb = a1 / (T(e.g1) - T(1)) + a2 / (T(e.g2) - T(1));
-  pb = a1 * T(e.g1) * T(e.p1) / (T(e.g1) - T(1)) + a2 * T(e.g2) * T(e.p2) / (T(e.g2) - T(1));
+  gb = dv(a1, T(e.g1) - T(1)) + dv(a2, T(e.g2) - T(1));
+  pb = dv(a1 * T(e.g1) * T(e.p1), T(e.g1) - T(1)) + dv(a2 * T(e.g2) * T(e.p2), T(e.g2) - T(1));
 }
 // mix_gamma_pinf SC:90-95
 template <typename T>
 __device__ __forceinline__ void mix_g_pi(const Eos& e, T a1, T& g, T& pi) {
   T gb, pb;
   mix_gb_pb(e, a1, gb, pb);
-  g = T(1) / gb + T(1);
-  pi = (g - T(1)) * pb / g;
+  g = dv(T(1), gb) + T(1);
+  pi = dv((g - T(1)) * pb, g);
 }
 // sound_speed SC:97-103
 template <typename T>
 __device__ __forceinline__ T sound_speed(const Eos& e, T rho, T p, T a1) {
   T g, pi;
   mix_g_pi(e, a1, g, pi);
-  T val = g * (p + pi) / rho;
+  T val = dv(g * (p + pi), rho);
   if (val < T(TINY)) val = T(TINY);
-  return sqrt(val);
+  return sq(val);
 }
 // prim_to_energy SC:105-110
 template <typename T>
@@ -184,9 +201,9 @@ __device__ __forceinline__ T thinc_l(T vm1, T v0, T v1, T t1) {
   if ((v1 - v0) * (v0 - vm1) > T(0)) {
     T qa = T(0.5) * (v1 + vm1);
     T qd = T(0.5) * (v1 - vm1);
-    T feng = (v0 - qa) / qd;
-    T t2 = tanh(feng * T(1.8) * T(0.5));
-    result = qa + qd * (t1 + t2 / t1) / (T(1) + t2);
+    T feng = dv(v0 - qa, qd);
+    T t2 = th(feng * T(1.8) * T(0.5));
+    result = qa + dv(qd * (t1 + dv(t2, t1)), T(1) + t2);
   }
   return result;
 }
@@ -197,9 +214,9 @@ __device__ __forceinline__ T thinc_r(T v0, T v1, T v2, T t1) {
   if ((v2 - v1) * (v1 - v0) > T(0)) {
     T qa = T(0.5) * (v2 + v0);
     T qd = T(0.5) * (v2 - v0);
-    T feng = (v1 - qa) / qd;
-    T t2 = tanh(feng * T(1.8) * T(0.5));
-    result = qa - qd * (t1 - t2 / t1) / (T(1) - t2);
+    T feng = dv(v1 - qa, qd);
+    T t2 = th(feng * T(1.8) * T(0.5));
+    result = qa - dv(qd * (t1 - dv(t2, t1)), T(1) - t2);
   }
   return result;
 }
@@ -284,14 +301,14 @@ template <int DIR, typename T>
 __device__ __forceinline__ Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg, T v_avg) {
   Basis<T> b;
   b.c = c;
-  b.inv2c = T(0.5) / c;
-  b.inv2c2 = T(0.5) / (c * c);
-  T Y1 = ar1 / rho;
-  T Y2 = ar2 / rho;
+  b.inv2c = dv(T(0.5), c);
+  b.inv2c2 = dv(T(0.5), c * c);
+  T Y1 = dv(ar1, rho);
+  T Y2 = dv(ar2, rho);
   b.Y1 = Y1;
   b.Y2 = Y2;
-  b.y1c2 = Y1 / (c * c);
-  b.y2c2 = Y2 / (c * c);
+  b.y1c2 = dv(Y1, c * c);
+  b.y2c2 = dv(Y2, c * c);
   T un = DIR == 0 ? u_avg : v_avg;
   b.k = un * b.inv2c;
   b.ut = DIR == 0 ? v_avg : -u_avg;  // to_char_x: ut = v;  to_char_y: ut_f = -u
@@ -460,6 +477,84 @@ __device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f,
   T right_part = nR + wmax(T(0), SR) * (ratR - T(1));
   T sp_sign = wsign(SP);
   ustar = T(0.5) * (T(1) + sp_sign) * left_part + T(0.5) * (T(1) - sp_sign) * right_part;
+}
+
+// HLLC for the mixed-precision path: wave speeds, contact speed and the
+// mass / momentum / volume-fraction fluxes in FP32; the energy flux in FP64.
+// E = gb p + pb + kin carries pb = gamma pi / (gamma - 1) (7765 for the
+// nondimensional water against p ~ 1), so an FP32 energy flux would put
+// ~eps32 * pb of noise into E and hence into the recovered pressure.
+template <int DIR>
+__device__ __forceinline__ void hllc_mixed(const Eos& e, const float* L, const float* R, double* f,
+                                           double& ustar) {
+  const float ar1L = L[0], ar2L = L[1], uL = L[2], vL = L[3], pL = L[4], a1L = L[5];
+  const float ar1R = R[0], ar2R = R[1], uR = R[2], vR = R[3], pR = R[4], a1R = R[5];
+  const float nL = DIR == 0 ? uL : vL, nR = DIR == 0 ? uR : vR;
+  float rhoL = ar1L + ar2L;
+  float rhoR = ar1R + ar2R;
+  float cL = sound_speed<float>(e, rhoL, pL, a1L);
+  float cR = sound_speed<float>(e, rhoR, pR, a1R);
+  float n_avg = 0.5f * (nL + nR);
+  float c_avg = 0.5f * (cL + cR);
+  float SL = wmin(nL - cL, n_avg - c_avg);
+  float SR = wmax(nR + cR, n_avg + c_avg);
+  float denom = rhoL * (SL - nL) - rhoR * (SR - nR);
+  if (fabsf(denom) < float(TINY)) denom = float(TINY);
+  float SP = dv(pR - pL + rhoL * nL * (SL - nL) - rhoR * nR * (SR - nR), denom);
+  float ratL = dv(SL - nL, SL - SP);
+  float ratR = dv(SR - nR, SR - SP);
+  // FP64 energies from the FP32 primitives
+  const double EL = prim_energy<double>(e, ar1L, ar2L, uL, vL, pL, a1L);
+  const double ER = prim_energy<double>(e, ar1R, ar2R, uR, vR, pR, a1R);
+  if (SL > 0.f) {
+    f[0] = ar1L * nL;
+    f[1] = ar2L * nL;
+    if (DIR == 0) { f[2] = rhoL * uL * uL + pL; f[3] = rhoL * uL * vL; }
+    else          { f[2] = rhoL * uL * vL;      f[3] = rhoL * vL * vL + pL; }
+    f[4] = (EL + (double)pL) * (double)nL;
+    f[5] = a1L * nL;
+  } else if (SL <= 0.f && 0.f < SP) {
+    float ratio = ratL;
+    f[0] = ar1L * nL + SL * (ar1L * ratio - ar1L);
+    f[1] = ar2L * nL + SL * (ar2L * ratio - ar2L);
+    if (DIR == 0) {
+      f[2] = (rhoL * uL * uL + pL) + SL * (rhoL * SP * ratio - rhoL * uL);
+      f[3] = (rhoL * uL * vL) + SL * (rhoL * vL * ratio - rhoL * vL);
+    } else {
+      f[2] = (rhoL * uL * vL) + SL * (rhoL * uL * ratio - rhoL * uL);
+      f[3] = (rhoL * vL * vL + pL) + SL * (rhoL * SP * ratio - rhoL * vL);
+    }
+    const double sl = SL, sp = SP, nl = nL, rl = rhoL, pl = pL;
+    const double EL_star = EL + (sp - nl) * (sp * rl + pl / (sl - nl));
+    f[4] = (EL + pl) * nl + sl * (EL_star * (double)ratio - EL);
+    f[5] = a1L * nL + SL * (a1L * ratio - a1L);
+  } else if (SP <= 0.f && 0.f <= SR) {
+    float ratio = ratR;
+    f[0] = ar1R * nR + SR * (ar1R * ratio - ar1R);
+    f[1] = ar2R * nR + SR * (ar2R * ratio - ar2R);
+    if (DIR == 0) {
+      f[2] = (rhoR * uR * uR + pR) + SR * (rhoR * SP * ratio - rhoR * uR);
+      f[3] = (rhoR * uR * vR) + SR * (rhoR * vR * ratio - rhoR * vR);
+    } else {
+      f[2] = (rhoR * uR * vR) + SR * (rhoR * uR * ratio - rhoR * uR);
+      f[3] = (rhoR * vR * vR + pR) + SR * (rhoR * SP * ratio - rhoR * vR);
+    }
+    const double sr = SR, sp = SP, nr = nR, rr = rhoR, pr = pR;
+    const double ER_star = ER + (sp - nr) * (sp * rr + pr / (sr - nr));
+    f[4] = (ER + pr) * nr + sr * (ER_star * (double)ratio - ER);
+    f[5] = a1R * nR + SR * (a1R * ratio - a1R);
+  } else {
+    f[0] = ar1R * nR;
+    f[1] = ar2R * nR;
+    if (DIR == 0) { f[2] = rhoR * uR * uR + pR; f[3] = rhoR * uR * vR; }
+    else          { f[2] = rhoR * uR * vR;      f[3] = rhoR * vR * vR + pR; }
+    f[4] = (ER + (double)pR) * (double)nR;
+    f[5] = a1R * nR;
+  }
+  float left_part = nL + wmin(0.f, SL) * (ratL - 1.f);
+  float right_part = nR + wmax(0.f, SR) * (ratR - 1.f);
+  float sp_sign = wsign(SP);
+  ustar = 0.5f * (1.f + sp_sign) * left_part + 0.5f * (1.f - sp_sign) * right_part;
 }
 
 }  // namespace wamp

@@ -112,12 +112,13 @@ __device__ __forceinline__ double ducros_sigma(const FaceSmem& S, int r) {
 // High-order flux through one face whose left cell is LDS index r0; cells
 // along the normal are r0 + k*s.  ci / sg: indicator and Ducros sensor at
 // the cells -1..2.  Writes 6 fluxes + u* to out.
-template <int DIR, int BASE, int SHEAR, int FACEPP>
+template <int DIR, int BASE, int SHEAR, int FACEPP, typename T>
 __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r0, const double* ci,
-                                          const double* sg, int vf_always, double t1, double* out) {
+                                          const double* sg, int vf_always, double t1d, double* out) {
   constexpr int s = DIR == 0 ? FRJ : 1;
   constexpr bool wide = BASE == 1 || SHEAR == 2;  // needs cells -2 and +3
   constexpr int klo = wide ? 0 : 1, khi = wide ? 6 : 5;
+  const T t1 = (T)t1d;
 
   bool use_thinc = wmin(wmin(ci[0], ci[1]), wmin(ci[2], ci[3])) < THINC_THRESH;
   bool use_smooth = false;
@@ -125,27 +126,27 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r
 
   // face-averaged state, SC:1039-1048
   const int rA = r0, rB = r0 + s;
-  double ar1a = 0.5 * (S.q[0][rA] + S.q[0][rB]);
-  double ar2a = 0.5 * (S.q[1][rA] + S.q[1][rB]);
-  double a1a = 0.5 * (S.q[5][rA] + S.q[5][rB]);
-  double pa = 0.5 * (S.q[4][rA] + S.q[4][rB]);
-  double rhoa = ar1a + ar2a;
-  double u_avg = 0.5 * (S.q[2][rA] + S.q[2][rB]) / rhoa;
-  double v_avg = 0.5 * (S.q[3][rA] + S.q[3][rB]) / rhoa;
-  double ca = sound_speed(e, rhoa, pa, a1a);
-  Basis<double> B = make_basis<DIR>(ca, rhoa, ar1a, ar2a, u_avg, v_avg);
+  T ar1a = T(0.5) * ((T)S.q[0][rA] + (T)S.q[0][rB]);
+  T ar2a = T(0.5) * ((T)S.q[1][rA] + (T)S.q[1][rB]);
+  T a1a = T(0.5) * ((T)S.q[5][rA] + (T)S.q[5][rB]);
+  T pa = T(0.5) * ((T)S.q[4][rA] + (T)S.q[4][rB]);
+  T rhoa = ar1a + ar2a;
+  T u_avg = dv(T(0.5) * ((T)S.q[2][rA] + (T)S.q[2][rB]), rhoa);
+  T v_avg = dv(T(0.5) * ((T)S.q[3][rA] + (T)S.q[3][rB]), rhoa);
+  T ca = sound_speed<T>(e, rhoa, pa, a1a);
+  Basis<T> B = make_basis<DIR>(ca, rhoa, ar1a, ar2a, u_avg, v_avg);
 
   // characteristic projection of the stencil cells -2..3 (SC:1051-1056)
-  double w[6][6];
+  T w[6][6];
 #pragma unroll
   for (int k = klo; k < khi; ++k) {
-    double u[6];
+    T u[6];
     const int r = r0 + (k - 2) * s;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) u[m] = S.q[m][r];
+    for (int m = 0; m < 6; ++m) u[m] = (T)S.q[m][r];
     to_char<DIR>(B, u, w[k]);
   }
-  double wl[6], wr[6];
+  T wl[6], wr[6];
 #pragma unroll
   for (int m = 0; m < 6; ++m) {
     if (BASE == 0) {
@@ -157,8 +158,8 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r
     }
   }
   if (SHEAR != 0 && use_smooth) {
-    double iph = SHEAR == 1 ? central4(w[1][3], w[2][3], w[3][3], w[4][3])
-                            : central6(w[0][3], w[1][3], w[2][3], w[3][3], w[4][3], w[5][3]);
+    T iph = SHEAR == 1 ? central4(w[1][3], w[2][3], w[3][3], w[4][3])
+                       : central6(w[0][3], w[1][3], w[2][3], w[3][3], w[4][3], w[5][3]);
     wl[3] = iph;
     wr[3] = iph;
   }
@@ -172,36 +173,46 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r
     wl[4] = thinc_l(w[1][4], w[2][4], w[3][4], t1);
     wr[4] = thinc_r(w[2][4], w[3][4], w[4][4], t1);
   }
-  double phL[6], phR[6], phfL[6], phfR[6];
+  T phL[6], phR[6], phfL[6], phfR[6];
   to_phys<DIR>(B, wl, phL);
   to_phys<DIR>(B, wr, phR);
   to_phys<DIR>(B, w[2], phfL);
   to_phys<DIR>(B, w[3], phfR);
 
   if (FACEPP == 0) {  // SC:1103-1111
-    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
-    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
-    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
+    if (phL[0] < T(0) || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < T(0) || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < T(0) || phL[5] > T(1)) phL[5] = phfL[5];
     if (isnan_(phL[4])) phL[4] = phfL[4];
-    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
-    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
-    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
+    if (phR[0] < T(0) || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < T(0) || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < T(0) || phR[5] > T(1)) phR[5] = phfR[5];
     if (isnan_(phR[4])) phR[4] = phfR[4];
-  } else {  // M10:1007-1021
-    pp_limit_state(e, phL, phfL);
-    pp_limit_state(e, phR, phfR);
-    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
-    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
-    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
-    if (rho_ct2_sc(e, phL) < PP_CTILDE || isnan_(phL[4])) phL[4] = phfL[4];
-    floor_pressure(e, phL);
-    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
-    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
-    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
-    if (rho_ct2_sc(e, phR) < PP_CTILDE || isnan_(phR[4])) phR[4] = phfR[4];
-    floor_pressure(e, phR);
+  } else {  // M10:1007-1021, thresholds always in FP64
+    double dL[6], dR[6], dfL[6], dfR[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      dL[m] = phL[m]; dR[m] = phR[m]; dfL[m] = phfL[m]; dfR[m] = phfR[m];
+    }
+    pp_limit_state(e, dL, dfL);
+    pp_limit_state(e, dR, dfR);
+    if (dL[0] < 0.0 || isnan_(dL[0])) dL[0] = dfL[0];
+    if (dL[1] < 0.0 || isnan_(dL[1])) dL[1] = dfL[1];
+    if (dL[5] < 0.0 || dL[5] > 1.0) dL[5] = dfL[5];
+    if (rho_ct2_sc(e, dL) < PP_CTILDE || isnan_(dL[4])) dL[4] = dfL[4];
+    floor_pressure(e, dL);
+    if (dR[0] < 0.0 || isnan_(dR[0])) dR[0] = dfR[0];
+    if (dR[1] < 0.0 || isnan_(dR[1])) dR[1] = dfR[1];
+    if (dR[5] < 0.0 || dR[5] > 1.0) dR[5] = dfR[5];
+    if (rho_ct2_sc(e, dR) < PP_CTILDE || isnan_(dR[4])) dR[4] = dfR[4];
+    floor_pressure(e, dR);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      phL[m] = (T)dL[m];
+      phR[m] = (T)dR[m];
+    }
   }
-  if (ca < 0.0 || isnan_(ca)) {
+  if (ca < T(0) || isnan_(ca)) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       phL[m] = phfL[m];
@@ -209,14 +220,18 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r
     }
   }
   // primitives for HLLC, SC:1118-1125
-  double rhoL = phL[0] + phL[1], rhoR = phR[0] + phR[1];
-  double PL[6] = {phL[0], phL[1], phL[2] / rhoL, phL[3] / rhoL, phL[4], phL[5]};
-  double PR[6] = {phR[0], phR[1], phR[2] / rhoR, phR[3] / rhoR, phR[4], phR[5]};
-  hllc<DIR>(e, PL, PR, out, out[6]);
+  T rhoL = phL[0] + phL[1], rhoR = phR[0] + phR[1];
+  T PL[6] = {phL[0], phL[1], dv(phL[2], rhoL), dv(phL[3], rhoL), phL[4], phL[5]};
+  T PR[6] = {phR[0], phR[1], dv(phR[2], rhoR), dv(phR[3], rhoR), phR[4], phR[5]};
+  if constexpr (sizeof(T) == sizeof(double)) {
+    hllc<DIR>(e, PL, PR, out, out[6]);
+  } else {
+    hllc_mixed<DIR>(e, PL, PR, out, out[6]);
+  }
 }
 
 // First-order flux (fx_low_kernel SC:1303-1347 / gy_low_kernel SC:1350-1394).
-template <int DIR>
+template <int DIR, typename T>
 __device__ __forceinline__ void face_low(const Eos& e, const FaceSmem& S, int r0, double* out) {
   constexpr int s = DIR == 0 ? FRJ : 1;
   const int rL = r0, rR = r0 + s;
@@ -235,12 +250,16 @@ __device__ __forceinline__ void face_low(const Eos& e, const FaceSmem& S, int r0
   mix_g_pi(e, aR, gR, piR);
   if (isnan_(pL) || (pL + piL) / (gL - 1.0) < PP_CTILDE) pL = PP_CTILDE * (gL - 1.0) - piL;
   if (isnan_(pR) || (pR + piR) / (gR - 1.0) < PP_CTILDE) pR = PP_CTILDE * (gR - 1.0) - piR;
-  double PL[6] = {S.q[0][rL], S.q[1][rL], uL, vL, pL, aL};
-  double PR[6] = {S.q[0][rR], S.q[1][rR], uR, vR, pR, aR};
-  hllc<DIR>(e, PL, PR, out, out[6]);
+  T PL[6] = {(T)S.q[0][rL], (T)S.q[1][rL], (T)uL, (T)vL, (T)pL, (T)aL};
+  T PR[6] = {(T)S.q[0][rR], (T)S.q[1][rR], (T)uR, (T)vR, (T)pR, (T)aR};
+  if constexpr (sizeof(T) == sizeof(double)) {
+    hllc<DIR>(e, PL, PR, out, out[6]);
+  } else {
+    hllc_mixed<DIR>(e, PL, PR, out, out[6]);
+  }
 }
 
-template <int BASE, int SHEAR, int FACEPP>
+template <int BASE, int SHEAR, int FACEPP, typename T>
 __global__ void __launch_bounds__(FTHREADS)
 k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
   __shared__ FaceSmem S;
@@ -297,10 +316,10 @@ k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
       ci[k] = S.cx[(ti + k) * FTJ + tj];
       sg[k] = SHEAR != 0 ? S.sx[(ti + k) * FTJ + tj] : 0.0;
     }
-    face_high<0, BASE, SHEAR, FACEPP>(e, S, r0, ci, sg, vf_always, t1, out);
+    face_high<0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[m][o] = out[m];
-    face_low<0>(e, S, r0, out);
+    face_low<0, T>(e, S, r0, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[14 + m][o] = out[m];
   }
@@ -311,10 +330,10 @@ k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
       ci[k] = S.cy[ti * (FTJ + 3) + tj + k];
       sg[k] = SHEAR != 0 ? S.sy[ti * (FTJ + 3) + tj + k] : 0.0;
     }
-    face_high<1, BASE, SHEAR, FACEPP>(e, S, r0, ci, sg, vf_always, t1, out);
+    face_high<1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[7 + m][o] = out[m];
-    face_low<1>(e, S, r0, out);
+    face_low<1, T>(e, S, r0, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[21 + m][o] = out[m];
   }
@@ -934,11 +953,11 @@ __global__ void __launch_bounds__(STHREADS) k_bc(Geo G, State U, double* up, dou
 // ========================================================================
 typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
 
-template <int BASE, int SHEAR, int FACEPP>
+template <int BASE, int SHEAR, int FACEPP, typename T>
 static void launch_faces_t(const Geo& G, const Eos& e, const State& U, const Faces& F,
                            int vf_always, double t1, hipStream_t st) {
   dim3 grid((G.ny + 1 + FTJ - 1) / FTJ, (G.nx + 1 + FTI - 1) / FTI);
-  hipLaunchKernelGGL((k_faces<BASE, SHEAR, FACEPP>), grid, dim3(FTHREADS), 0, st, G, e, U, F,
+  hipLaunchKernelGGL((k_faces<BASE, SHEAR, FACEPP, T>), grid, dim3(FTHREADS), 0, st, G, e, U, F,
                      vf_always, t1);
 }
 
@@ -950,11 +969,14 @@ using namespace wamp;
 
 hipError_t wamp_launch_faces(const Geo* G, const Eos* e, const State* U, const Faces* F,
                              int base, int shear, int face_pp, int vf_always, double t1,
-                             hipStream_t st) {
-#define WAMP_F(B, S, P)                                                  \
-  if (base == B && shear == S && face_pp == P) {                        \
-    launch_faces_t<B, S, P>(*G, *e, *U, *F, vf_always, t1, st);         \
-    return hipGetLastError();                                           \
+                             int precision, hipStream_t st) {
+#define WAMP_F(B, S, P)                                                      \
+  if (base == B && shear == S && face_pp == P) {                            \
+    if (precision == 0)                                                     \
+      launch_faces_t<B, S, P, double>(*G, *e, *U, *F, vf_always, t1, st);   \
+    else                                                                    \
+      launch_faces_t<B, S, P, float>(*G, *e, *U, *F, vf_always, t1, st);    \
+    return hipGetLastError();                                               \
   }
   WAMP_F(0, 0, 0) WAMP_F(0, 0, 1) WAMP_F(0, 1, 0) WAMP_F(0, 1, 1) WAMP_F(0, 2, 0) WAMP_F(0, 2, 1)
   WAMP_F(1, 0, 0) WAMP_F(1, 0, 1) WAMP_F(1, 1, 0) WAMP_F(1, 1, 1) WAMP_F(1, 2, 0) WAMP_F(1, 2, 1)
