@@ -120,6 +120,44 @@ int wamp_get_dt(wamp_ctx *ctx, double *const U[6], double *dt_dev, void *stream)
 int wamp_step(wamp_ctx *ctx, double *const U[6], const double *dt_dev, double *dt_next_dev,
               void *stream);
 
+/* ---- Batches ------------------------------------------------------------
+   No counterpart in the reference (one case per script run).  A batch
+   advances many independent cases -- any grids and EOS, one scheme and
+   precision -- with one persistent launch per kernel kind per RK stage over
+   a work list of every case's face tiles / limiter rows.  Per-case dt, time,
+   end time and a finished flag stay on the device; a step can be captured in
+   a hipGraph.  Cases are held in slots; the caller owns each case's state
+   arrays, the batch owns each case's scratch (face and stage buffers). */
+typedef struct wamp_batch wamp_batch;
+
+/* scheme must be SC (base 0, shear 1, face_pp 0), M10 (0, 0, 1) or MP5 (1, 2, 0). */
+int wamp_batch_create(const wamp_scheme *scheme, int32_t precision, int32_t max_cases,
+                      wamp_batch **out);
+int wamp_batch_destroy(wamp_batch *b);
+/* Put a case in a slot (allocates its scratch); t_end <= 0: no end time. */
+int wamp_batch_set_case(wamp_batch *b, int32_t slot, const wamp_grid *grid, const wamp_eos *eos,
+                        double *const U[6], double t0, double t_end);
+int wamp_batch_clear_case(wamp_batch *b, int32_t slot);
+/* Work decomposition: tile 0 = 8x32 face tiles, 1 = 16x16; persistent blocks
+   per CU for the face and limiter kernels; limiter rows per wave (8/16/32).
+   Results do not depend on these. */
+int wamp_batch_set_tuning(wamp_batch *b, int32_t tile, int32_t faces_blocks_per_cu,
+                          int32_t lu_blocks_per_cu, int32_t lu_seg);
+/* Upload the case table and work lists (after set/clear/tuning). */
+int wamp_batch_commit(wamp_batch *b, void *stream);
+/* Initial sequence and first dt of the given slots (all occupied if n_slots < 0). */
+int wamp_batch_prepare(wamp_batch *b, const int32_t *slots, int32_t n_slots, void *stream);
+int wamp_batch_set_done(wamp_batch *b, int32_t slot, int32_t done, void *stream);
+/* Capture `steps` steps in a hipGraph (0 drops it). */
+int wamp_batch_graph(wamp_batch *b, int32_t steps);
+/* One step (or one graph replay) of every unfinished case; returns the number
+   of steps enqueued (> 0) or an error (< 0 / hipError_t). */
+int wamp_batch_step(wamp_batch *b, void *stream);
+/* Per-slot time, dt and finished flag to the host (synchronises the stream). */
+int wamp_batch_status(wamp_batch *b, double *t, double *dt, int32_t *done, void *stream);
+/* Work-list lengths: which = 0 face tiles, 1 limiter wave items. */
+int64_t wamp_batch_work(const wamp_batch *b, int32_t which);
+
 /* Measurement hook (no counterpart in the reference; for benchmarks and
    profilers).  wamp_timing_enable(ctx, n) makes the next n calls of
    wamp_step record hipEvents on their stream around each stage's face-flux

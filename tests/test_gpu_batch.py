@@ -1,0 +1,104 @@
+"""Batched solves: every case is bitwise the same as solved alone, whatever the
+tuning, with or without a captured graph; end times freeze a case exactly."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(precision):
+    from wamp import cases
+    from wamp.config import SCHEME_MP5
+
+    rng = np.random.default_rng(11)
+    out = [cases.shock_cylinder(96, 48, scheme=SCHEME_MP5),
+           cases.triple_point(70, 30, scheme=SCHEME_MP5),
+           cases.batch_case(rng, scheme=SCHEME_MP5, grids=[(80, 64)]),
+           cases.batch_case(rng, scheme=SCHEME_MP5, grids=[(130, 40)])]
+    return [(dataclasses.replace(c, precision=precision), U) for c, U in out]
+
+
+def _alone(cfg, U0, steps):
+    from wamp.solver import Solver
+
+    s = Solver(cfg)
+    s.set_state(U0)
+    s.prepare()
+    s.run(steps)
+    out = s.state()
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_alone_under_all_tunings(gpu_ok, precision):
+    from wamp.batch import Batch, Tuning
+
+    cs = _cases(precision)
+    steps = 6
+    ref = [_alone(c, U, steps) for c, U in cs]
+    for tuning, graph in ((Tuning(0, 3, 2, 16), 0), (Tuning(1, 2, 4, 8), 3), (Tuning(0, 1, 1, 32), 6)):
+        b = Batch(cs[0][0].scheme, precision, max_cases=6)
+        for k, (c, U) in enumerate(cs):
+            b.set_case(k + 1, c, U)  # slot 0 and 5 stay empty
+        b.set_tuning(tuning)
+        b.commit()
+        b.prepare()
+        if graph:
+            b.capture(graph)
+        done = 0
+        while done < steps:
+            done += b.step()
+        assert done == steps
+        for k, (c, _) in enumerate(cs):
+            si, sj = c.grid.interior()
+            got = b.state(k + 1)
+            assert np.array_equal(got[:, si, sj], ref[k][:, si, sj]), (tuning, graph, k)
+        b.close()
+
+
+def test_end_time_is_hit_exactly_and_freezes_the_case(gpu_ok):
+    from wamp.batch import Batch
+
+    cs = _cases(1)
+    b = Batch(cs[0][0].scheme, 1, max_cases=4)
+    t_end = [3e-3, 0.0, 1e-3, 2e-3]
+    for k, (c, U) in enumerate(cs):
+        b.set_case(k, c, U, t0=0.0, t_end=t_end[k])
+    b.commit()
+    b.prepare()
+    b.capture(4)
+    for _ in range(100):
+        b.step()
+        t, dt, done = b.status()
+        if done[0] and done[2] and done[3]:
+            break
+    assert done[0] and not done[1] and done[2] and done[3]
+    for k in (0, 2, 3):
+        assert t[k] == pytest.approx(t_end[k], rel=1e-12)
+    frozen = b.state(2)
+    b.step()
+    assert np.array_equal(b.state(2), frozen)
+    b.close()
+
+
+def test_autotune_keeps_results(gpu_ok):
+    from wamp.batch import Batch, Tuning
+
+    cs = _cases(1)
+    ref = [_alone(c, U, 4) for c, U in cs]
+    b = Batch(cs[0][0].scheme, 1, max_cases=len(cs))
+    for k, (c, U) in enumerate(cs):
+        b.set_case(k, c, U)
+    b.commit()
+    b.prepare()
+    best = b.autotune(steps=1, space=[Tuning(0, 3, 2, 16), Tuning(1, 2, 2, 8)])
+    assert best in (Tuning(0, 3, 2, 16), Tuning(1, 2, 2, 8))
+    for _ in range(4):
+        b.step()
+    for k, (c, _) in enumerate(cs):
+        si, sj = c.grid.interior()
+        assert np.array_equal(b.state(k)[:, si, sj], ref[k][:, si, sj])
+    b.close()

@@ -10,27 +10,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "wamp.h"
 
 namespace wamp {
-struct Geo {
-  int nx, ny, g, ld;
-  double dx, dy;
-  int bc[4];
-};
 struct Eos {
   double g1, p1, g2, p2;
 };
-struct State {
-  double* f[6];
-};
-struct Faces {
-  double* f[28];
-};
 }  // namespace wamp
+#include "wamp_types.h"
 
 using namespace wamp;
 
@@ -55,6 +47,17 @@ hipError_t wamp_launch_enforce(const Geo*, const Eos*, const State*, hipStream_t
 hipError_t wamp_launch_cons_to_prim(const Geo*, const Eos*, const State*, double*, double*,
                                     double*, hipStream_t);
 hipError_t wamp_launch_bc(const Geo*, const State*, double*, double*, double*, hipStream_t);
+hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int ncase,
+                                   const BatchScal* sc, int stage, int base, int shear,
+                                   int face_pp, int vf_always, double t1, int precision, int tile,
+                                   int blocks, hipStream_t st);
+hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int ncase,
+                                const BatchScal* sc, int stage, int repair, int seg, int blocks,
+                                hipStream_t st);
+hipError_t wamp_launch_batch_finalize(const CaseDesc* cs, int ncase, const BatchScal* sc,
+                                      hipStream_t st);
+hipError_t wamp_launch_batch_dt0(const Geo* G, const Eos* e, const State* U, double* acc,
+                                 const BatchScal* sc, int k, double cfl, hipStream_t st);
 }
 
 struct wamp_ctx {
@@ -367,6 +370,343 @@ int wamp_timing_read(wamp_ctx* c, double ms[2], int64_t launches[2]) {
   }
   c->timing_n = 0;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct wamp_batch {
+  wamp_scheme scheme;
+  int precision;
+  int max_cases;
+  int device;
+  int cu_count;
+  double t1;
+  // tuning
+  int tile;            // 0: 8x32 face tiles, 1: 16x16
+  int faces_bpcu;      // persistent face blocks per CU
+  int lu_bpcu;         // persistent limiter blocks (4 waves) per CU
+  int lu_seg;          // limiter rows per wave
+  // host mirror of the case table
+  std::vector<CaseDesc> desc;
+  std::vector<int> used;         // slot occupied
+  std::vector<double*> scratch;  // per-slot scratch allocation
+  std::vector<double> t0, tend;
+  // device side
+  CaseDesc* desc_dev;
+  int* off_dev;   // tile offsets [max+1] | item offsets [max+1]
+  double* scal;   // dt | acc | t | tend, max_cases each
+  int* done;
+  BatchScal sc;
+  int committed;
+  int total_tiles, total_items;
+  // graph
+  hipStream_t cap_stream;
+  hipGraphExec_t graph;
+  int graph_steps;
+};
+
+static const int TILE_TI[2] = {8, 16};
+static const int TILE_TJ[2] = {32, 16};
+static const int MJ_LANES = 58;
+
+static void case_work(wamp_batch* b, CaseDesc& d) {
+  const int ti = TILE_TI[b->tile], tj = TILE_TJ[b->tile];
+  d.tiles_j = (d.geo.ny + 1 + tj - 1) / tj;
+  d.tiles = d.tiles_j * ((d.geo.nx + 1 + ti - 1) / ti);
+  d.strips = (d.geo.ny + MJ_LANES - 1) / MJ_LANES;
+  d.items = d.strips * ((d.geo.nx + b->lu_seg - 1) / b->lu_seg);
+}
+
+static void graph_drop(wamp_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+extern "C" {
+
+int wamp_batch_create(const wamp_scheme* scheme, int32_t precision, int32_t max_cases,
+                      wamp_batch** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (!scheme) return fail(-1, "scheme is NULL");
+  if (max_cases < 1 || max_cases > 4096) return fail(-3, "max_cases must be in [1, 4096]");
+  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
+    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
+  const int b_ = scheme->base, s_ = scheme->shear, p_ = scheme->face_pp;
+  if (!((b_ == 0 && s_ == 1 && p_ == 0) || (b_ == 0 && s_ == 0 && p_ == 1) ||
+        (b_ == 1 && s_ == 2 && p_ == 0)))
+    return fail(-3, "batches support the SC (0,1,0), M10 (0,0,1) and MP5 (1,2,0) "
+                    "(base, shear, face_pp) schemes");
+  if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
+  wamp_batch* b = new (std::nothrow) wamp_batch();
+  if (!b) return fail(-5, "host allocation failed");
+  b->scheme = *scheme;
+  b->precision = precision;
+  b->max_cases = max_cases;
+  b->t1 = std::tanh(1.8 * 0.5);
+  b->tile = 0;
+  b->faces_bpcu = 3;
+  b->lu_bpcu = 2;
+  b->lu_seg = 16;
+  b->desc.assign(max_cases, CaseDesc());
+  b->used.assign(max_cases, 0);
+  b->scratch.assign(max_cases, nullptr);
+  b->t0.assign(max_cases, 0.0);
+  b->tend.assign(max_cases, 0.0);
+  int rc = hip_check(hipGetDevice(&b->device), "hipGetDevice");
+  if (!rc) rc = hip_check(hipDeviceGetAttribute(&b->cu_count, hipDeviceAttributeMultiprocessorCount,
+                                                b->device), "hipDeviceGetAttribute");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->desc_dev, sizeof(CaseDesc) * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->off_dev, sizeof(int) * 2 * (max_cases + 1)), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->scal, sizeof(double) * 4 * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->done, sizeof(int) * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  b->sc.dt = b->scal;
+  b->sc.acc = b->scal + max_cases;
+  b->sc.t = b->scal + 2 * max_cases;
+  b->sc.tend = b->scal + 3 * max_cases;
+  b->sc.done = b->done;
+  std::vector<double> init(4 * max_cases, 0.0);
+  for (int k = 0; k < max_cases; ++k) init[max_cases + k] = 1.0e30;
+  std::vector<int> ones(max_cases, 1);
+  rc = hip_check(hipMemcpy(b->scal, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice), "hipMemcpy");
+  if (!rc) rc = hip_check(hipMemcpy(b->done, ones.data(), sizeof(int) * max_cases, hipMemcpyHostToDevice), "hipMemcpy");
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int wamp_batch_destroy(wamp_batch* b) {
+  if (!b) return 0;
+  graph_drop(b);
+  for (double* p : b->scratch)
+    if (p) (void)hipFree(p);
+  (void)hipFree(b->desc_dev);
+  (void)hipFree(b->off_dev);
+  (void)hipFree(b->scal);
+  (void)hipFree(b->done);
+  (void)hipStreamDestroy(b->cap_stream);
+  delete b;
+  return 0;
+}
+
+int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, const wamp_eos* eos,
+                        double* const U[6], double t0, double t_end) {
+  if (!b) return fail(-1, "NULL batch");
+  if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
+  if (!grid || !eos) return fail(-1, "grid and eos must be non-NULL");
+  if (grid->nx < 1 || grid->ny < 1 || grid->g < 4 || grid->ld < grid->ny + 2 * grid->g ||
+      !(grid->dx > 0.0) || !(grid->dy > 0.0))
+    return fail(-3, "invalid grid for slot %d", slot);
+  if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  State st;
+  if (int rc = state_of(U, st, "U")) return rc;
+  if (b->scratch[slot]) {
+    (void)hipFree(b->scratch[slot]);
+    b->scratch[slot] = nullptr;
+  }
+  const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
+  double* p = nullptr;
+  if (int rc = hip_check(hipMalloc((void**)&p, 40 * plane * sizeof(double)), "hipMalloc(case scratch)"))
+    return rc;
+  b->scratch[slot] = p;
+  CaseDesc& d = b->desc[slot];
+  d = CaseDesc();
+  d.geo = Geo{grid->nx, grid->ny, grid->g, grid->ld, grid->dx, grid->dy,
+              {grid->bc[0], grid->bc[1], grid->bc[2], grid->bc[3]}};
+  d.eos = Eos{eos->gamma1, eos->pinf1, eos->gamma2, eos->pinf2};
+  for (int m = 0; m < 6; ++m) d.u[0][m] = U[m];
+  for (int m = 0; m < 28; ++m, p += plane) d.f[m] = p;
+  for (int m = 0; m < 6; ++m, p += plane) d.u[1][m] = p;
+  for (int m = 0; m < 6; ++m, p += plane) d.u[2][m] = p;
+  d.cfl = b->scheme.cfl;
+  b->used[slot] = 1;
+  b->t0[slot] = t0;
+  b->tend[slot] = t_end;
+  b->committed = 0;
+  return 0;
+}
+
+int wamp_batch_clear_case(wamp_batch* b, int32_t slot) {
+  if (!b) return fail(-1, "NULL batch");
+  if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
+  if (b->scratch[slot]) (void)hipFree(b->scratch[slot]);
+  b->scratch[slot] = nullptr;
+  b->used[slot] = 0;
+  b->desc[slot] = CaseDesc();
+  b->committed = 0;
+  return 0;
+}
+
+int wamp_batch_set_tuning(wamp_batch* b, int32_t tile, int32_t faces_bpcu, int32_t lu_bpcu,
+                          int32_t lu_seg) {
+  if (!b) return fail(-1, "NULL batch");
+  if (tile < 0 || tile > 1) return fail(-3, "tile must be 0 (8x32) or 1 (16x16)");
+  if (faces_bpcu < 1 || faces_bpcu > 16 || lu_bpcu < 1 || lu_bpcu > 16)
+    return fail(-3, "blocks per CU must be in [1, 16]");
+  if (lu_seg != 8 && lu_seg != 16 && lu_seg != 32) return fail(-3, "lu_seg must be 8, 16 or 32");
+  b->tile = tile;
+  b->faces_bpcu = faces_bpcu;
+  b->lu_bpcu = lu_bpcu;
+  b->lu_seg = lu_seg;
+  b->committed = 0;
+  return 0;
+}
+
+// Upload the case table and work-list offsets; a slot's time and end time
+// are (re)initialised only when its case was (re)set since the last commit.
+int wamp_batch_commit(wamp_batch* b, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  hipStream_t st = (hipStream_t)stream;
+  graph_drop(b);
+  const int n = b->max_cases;
+  std::vector<int> off(2 * (n + 1), 0);
+  for (int k = 0; k < n; ++k) {
+    int tiles = 0, items = 0;
+    if (b->used[k]) {
+      case_work(b, b->desc[k]);
+      tiles = b->desc[k].tiles;
+      items = b->desc[k].items;
+    }
+    off[k + 1] = off[k] + tiles;
+    off[n + 1 + k + 1] = off[n + 1 + k] + items;
+  }
+  b->total_tiles = off[n];
+  b->total_items = off[2 * n + 1];
+  if (int rc = hip_check(hipMemcpyAsync(b->desc_dev, b->desc.data(), sizeof(CaseDesc) * n,
+                                        hipMemcpyHostToDevice, st), "hipMemcpyAsync(desc)")) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(b->off_dev, off.data(), sizeof(int) * off.size(),
+                                        hipMemcpyHostToDevice, st), "hipMemcpyAsync(offsets)")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+  b->committed = 1;
+  return 0;
+}
+
+// Initial sequence (repair + clamp, SC:1998-2001) and first dt of the slots
+// listed in `slots` (all occupied slots when n_slots < 0); sets t = t0.
+int wamp_batch_prepare(wamp_batch* b, const int32_t* slots, int32_t n_slots, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int> list;
+  if (n_slots < 0) {
+    for (int k = 0; k < b->max_cases; ++k)
+      if (b->used[k]) list.push_back(k);
+  } else {
+    if (!slots && n_slots > 0) return fail(-2, "slots is NULL");
+    for (int q = 0; q < n_slots; ++q) {
+      if (slots[q] < 0 || slots[q] >= b->max_cases || !b->used[slots[q]])
+        return fail(-3, "slot %d is not occupied", slots[q]);
+      list.push_back(slots[q]);
+    }
+  }
+  for (int k : list) {
+    const CaseDesc& d = b->desc[k];
+    State s;
+    for (int m = 0; m < 6; ++m) s.f[m] = d.u[0][m];
+    double vals[2] = {b->t0[k], b->tend[k]};
+    if (int rc = hip_check(hipMemcpyAsync(b->sc.t + k, &vals[0], sizeof(double), hipMemcpyHostToDevice, st), "t0")) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(b->sc.tend + k, &vals[1], sizeof(double), hipMemcpyHostToDevice, st), "tend")) return rc;
+    if (int rc = hip_check(wamp_launch_prepare(&d.geo, &d.eos, &s, b->scheme.repair, st), "prepare")) return rc;
+    if (int rc = hip_check(wamp_launch_batch_dt0(&d.geo, &d.eos, &s, b->sc.acc + k, &b->sc, k, d.cfl, st), "dt0")) return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;  // vals lifetime
+  }
+  return 0;
+}
+
+// Mark slots finished (frozen) or not.
+int wamp_batch_set_done(wamp_batch* b, int32_t slot, int32_t done, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
+  int v = done ? 1 : 0;
+  if (int rc = hip_check(hipMemcpyAsync(b->done + slot, &v, sizeof(int), hipMemcpyHostToDevice,
+                                        (hipStream_t)stream), "done")) return rc;
+  return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+static int batch_enqueue_step(wamp_batch* b, hipStream_t st) {
+  const int n = b->max_cases;
+  const int fb = std::max(1, std::min(b->total_tiles, b->faces_bpcu * b->cu_count));
+  const int lb = std::max(1, std::min((b->total_items + 3) / 4, b->lu_bpcu * b->cu_count));
+  for (int s = 0; s < 3; ++s) {
+    if (int rc = hip_check(wamp_launch_faces_batch(b->desc_dev, b->off_dev, n, &b->sc, s,
+                                                   b->scheme.base, b->scheme.shear,
+                                                   b->scheme.face_pp, b->scheme.vf_thinc, b->t1,
+                                                   b->precision, b->tile, fb, st), "faces_batch"))
+      return rc;
+    if (int rc = hip_check(wamp_launch_lu_batch(b->desc_dev, b->off_dev + n + 1, n, &b->sc, s,
+                                                b->scheme.repair, b->lu_seg, lb, st), "lu_batch"))
+      return rc;
+  }
+  return hip_check(wamp_launch_batch_finalize(b->desc_dev, n, &b->sc, st), "finalize");
+}
+
+// Capture `steps` consecutive steps in a hipGraph (on the batch's own
+// capture stream); wamp_batch_step then replays it.  0 drops the graph.
+int wamp_batch_graph(wamp_batch* b, int32_t steps) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  graph_drop(b);
+  if (steps <= 0) return 0;
+  hipGraph_t g = nullptr;
+  if (int rc = hip_check(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal),
+                         "hipStreamBeginCapture")) return rc;
+  int rc = 0;
+  for (int k = 0; k < steps && !rc; ++k) rc = batch_enqueue_step(b, b->cap_stream);
+  int rc2 = hip_check(hipStreamEndCapture(b->cap_stream, &g), "hipStreamEndCapture");
+  if (rc || rc2) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : rc2;
+  }
+  rc = hip_check(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  if (rc) {
+    b->graph = nullptr;
+    return rc;
+  }
+  b->graph_steps = steps;
+  return 0;
+}
+
+// Advance every occupied, unfinished case: with a captured graph, one replay
+// (graph_steps steps); otherwise one step.  Returns the steps enqueued (> 0).
+int wamp_batch_step(wamp_batch* b, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->graph) {
+    if (int rc = hip_check(hipGraphLaunch(b->graph, st), "hipGraphLaunch")) return rc;
+    return b->graph_steps;
+  }
+  if (int rc = batch_enqueue_step(b, st)) return rc;
+  return 1;
+}
+
+// Copy t, dt and done of every slot to the host (synchronises the stream).
+int wamp_batch_status(wamp_batch* b, double* t, double* dt, int32_t* done, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = b->max_cases;
+  if (t && hip_check(hipMemcpyAsync(t, b->sc.t, sizeof(double) * n, hipMemcpyDeviceToHost, st), "t")) return -7;
+  if (dt && hip_check(hipMemcpyAsync(dt, b->sc.dt, sizeof(double) * n, hipMemcpyDeviceToHost, st), "dt")) return -7;
+  if (done && hip_check(hipMemcpyAsync(done, b->done, sizeof(int) * n, hipMemcpyDeviceToHost, st), "done")) return -7;
+  return hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+int64_t wamp_batch_work(const wamp_batch* b, int32_t which) {
+  if (!b) return -1;
+  return which == 0 ? b->total_tiles : b->total_items;
 }
 
 }  // extern "C"

@@ -27,20 +27,10 @@
 #include <stdint.h>
 
 #include "wamp_device.h"
+#include "wamp_types.h"
 
 namespace wamp {
 
-struct Geo {
-  int nx, ny, g, ld;
-  double dx, dy;
-  int bc[4];
-};
-struct State {
-  double* f[6];
-};
-struct Faces {
-  double* f[28];
-};
 
 // ------------------------------------------------------------------------
 // Boundary fill by index mapping.  Ghost cell (i, j) holds the edge
@@ -72,28 +62,36 @@ __device__ __forceinline__ size_t map_cell(const Geo& G, int i, int j, double& s
 // ========================================================================
 // k_faces
 // ========================================================================
-constexpr int FTI = 8;             // face positions per tile along i
-constexpr int FTJ = 32;            // ... along j (contiguous)
-constexpr int FRI = FTI + 7;       // LDS rectangle: cells i0-3 .. i0+FTI+3
-constexpr int FRJ = FTJ + 7;
-constexpr int FRN = FRI * FRJ;
-constexpr int FCXN = (FTI + 3) * FTJ;  // x-indicator cells: i0-1..i0+FTI+1, j0..j0+FTJ-1
-constexpr int FCYN = FTI * (FTJ + 3);  // y-indicator cells: i0..i0+FTI-1, j0-1..j0+FTJ+1
-constexpr int FTHREADS = FTI * FTJ;
+// Tile of TI x TJ face positions (j contiguous), 256 threads.
+template <int TI_, int TJ_>
+struct FTile {
+  static constexpr int TI = TI_, TJ = TJ_;
+  static constexpr int RI = TI + 7, RJ = TJ + 7;  // LDS rectangle: cells i0-3 .. i0+TI+3
+  static constexpr int RN = RI * RJ;
+  static constexpr int CXN = (TI + 3) * TJ;       // x-indicator cells i0-1..i0+TI+1, j0..j0+TJ-1
+  static constexpr int CYN = TI * (TJ + 3);       // y-indicator cells i0..i0+TI-1, j0-1..j0+TJ+1
+  static constexpr int THREADS = TI * TJ;
+};
+using Tile8x32 = FTile<8, 32>;
+using Tile16x16 = FTile<16, 16>;
+constexpr int FTHREADS = 256;
 
+template <class TL>
 struct FaceSmem {
-  double q[6][FRN];  // ar1, ar2, ru, rv, p, a1
-  double ent[FRN];
-  double cx[FCXN], cy[FCYN];
-  double sx[FCXN], sy[FCYN];
+  double q[6][TL::RN];  // ar1, ar2, ru, rv, p, a1
+  double ent[TL::RN];
+  double cx[TL::CXN], cy[TL::CYN];
+  double sx[TL::CXN], sy[TL::CYN];
 };
 
 // Ducros x pressure sensor (ducros_kernel SC:875-916) at LDS cell r.
-__device__ __forceinline__ double lds_vel(const FaceSmem& S, int r, int comp) {
+template <class TL>
+__device__ __forceinline__ double lds_vel(const FaceSmem<TL>& S, int r, int comp) {
   return S.q[comp][r] / (S.q[0][r] + S.q[1][r]);
 }
-template <int DIR>
-__device__ __forceinline__ double ducros_sigma(const FaceSmem& S, int r) {
+template <class TL, int DIR>
+__device__ __forceinline__ double ducros_sigma(const FaceSmem<TL>& S, int r) {
+  constexpr int FRJ = TL::RJ;
   double der_ux = 0.5 * (-lds_vel(S, r - FRJ, 2) + lds_vel(S, r + FRJ, 2));
   double der_vx = 0.5 * (-lds_vel(S, r - FRJ, 3) + lds_vel(S, r + FRJ, 3));
   double der_uy = 0.5 * (-lds_vel(S, r - 1, 2) + lds_vel(S, r + 1, 2));
@@ -112,10 +110,10 @@ __device__ __forceinline__ double ducros_sigma(const FaceSmem& S, int r) {
 // High-order flux through one face whose left cell is LDS index r0; cells
 // along the normal are r0 + k*s.  ci / sg: indicator and Ducros sensor at
 // the cells -1..2.  Writes 6 fluxes + u* to out.
-template <int DIR, int BASE, int SHEAR, int FACEPP, typename T>
-__device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r0, const double* ci,
+template <class TL, int DIR, int BASE, int SHEAR, int FACEPP, typename T>
+__device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, int r0, const double* ci,
                                           const double* sg, int vf_always, double t1d, double* out) {
-  constexpr int s = DIR == 0 ? FRJ : 1;
+  constexpr int s = DIR == 0 ? TL::RJ : 1;
   constexpr bool wide = BASE == 1 || SHEAR == 2;  // needs cells -2 and +3
   constexpr int klo = wide ? 0 : 1, khi = wide ? 6 : 5;
   const T t1 = (T)t1d;
@@ -231,9 +229,9 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem& S, int r
 }
 
 // First-order flux (fx_low_kernel SC:1303-1347 / gy_low_kernel SC:1350-1394).
-template <int DIR, typename T>
-__device__ __forceinline__ void face_low(const Eos& e, const FaceSmem& S, int r0, double* out) {
-  constexpr int s = DIR == 0 ? FRJ : 1;
+template <class TL, int DIR, typename T>
+__device__ __forceinline__ void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) {
+  constexpr int s = DIR == 0 ? TL::RJ : 1;
   const int rL = r0, rR = r0 + s;
   double rhoL = S.q[0][rL] + S.q[1][rL];
   double rhoR = S.q[0][rR] + S.q[1][rR];
@@ -259,13 +257,16 @@ __device__ __forceinline__ void face_low(const Eos& e, const FaceSmem& S, int r0
   }
 }
 
-template <int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(FTHREADS)
-k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
-  __shared__ FaceSmem S;
+// One tile of face positions (bi, bj) of one case: stage, indicators, faces.
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const State& U, const Faces& F,
+                                           int vf_always, double t1, int bi, int bj,
+                                           FaceSmem<TL>& S) {
+  constexpr int FTI = TL::TI, FTJ = TL::TJ, FRJ = TL::RJ, FRN = TL::RN;
+  constexpr int FCXN = TL::CXN, FCYN = TL::CYN;
   const int tid = threadIdx.x;
-  const int i0 = G.g - 1 + (int)blockIdx.y * FTI;
-  const int j0 = G.g - 1 + (int)blockIdx.x * FTJ;
+  const int i0 = G.g - 1 + bi * FTI;
+  const int j0 = G.g - 1 + bj * FTJ;
 
   // Phase A: stage the cells of the rectangle, pressure and entropy once.
   for (int r = tid; r < FRN; r += FTHREADS) {
@@ -295,12 +296,12 @@ k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
   for (int k = tid; k < FCXN; k += FTHREADS) {
     const int r = (k / FTJ + 2) * FRJ + (k % FTJ + 3);
     S.cx[k] = cator2(S.ent[r - 2 * FRJ], S.ent[r - FRJ], S.ent[r], S.ent[r + FRJ], S.ent[r + 2 * FRJ]);
-    if (SHEAR != 0) S.sx[k] = ducros_sigma<0>(S, r);
+    if (SHEAR != 0) S.sx[k] = ducros_sigma<TL, 0>(S, r);
   }
   for (int k = tid; k < FCYN; k += FTHREADS) {
     const int r = (k / (FTJ + 3) + 3) * FRJ + (k % (FTJ + 3) + 2);
     S.cy[k] = cator2(S.ent[r - 2], S.ent[r - 1], S.ent[r], S.ent[r + 1], S.ent[r + 2]);
-    if (SHEAR != 0) S.sy[k] = ducros_sigma<1>(S, r);
+    if (SHEAR != 0) S.sy[k] = ducros_sigma<TL, 1>(S, r);
   }
   __syncthreads();
 
@@ -316,10 +317,10 @@ k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
       ci[k] = S.cx[(ti + k) * FTJ + tj];
       sg[k] = SHEAR != 0 ? S.sx[(ti + k) * FTJ + tj] : 0.0;
     }
-    face_high<0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
+    face_high<TL, 0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[m][o] = out[m];
-    face_low<0, T>(e, S, r0, out);
+    face_low<TL, 0, T>(e, S, r0, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[14 + m][o] = out[m];
   }
@@ -330,13 +331,21 @@ k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
       ci[k] = S.cy[ti * (FTJ + 3) + tj + k];
       sg[k] = SHEAR != 0 ? S.sy[ti * (FTJ + 3) + tj + k] : 0.0;
     }
-    face_high<1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
+    face_high<TL, 1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[7 + m][o] = out[m];
-    face_low<1, T>(e, S, r0, out);
+    face_low<TL, 1, T>(e, S, r0, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) F.f[21 + m][o] = out[m];
   }
+}
+
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(FTHREADS)
+k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
+  __shared__ FaceSmem<TL> S;
+  faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, (int)blockIdx.y,
+                                         (int)blockIdx.x, S);
 }
 
 // ========================================================================
@@ -584,18 +593,20 @@ __device__ __forceinline__ double blend3(double H, double L, const double* th, i
   return f;
 }
 
+// One wave's work item: columns strip*58-3 .. +64, rows seg*SEG .. +SEG.
+// Returns this lane's CFL minimum of the updated cells (1e30 if none or if
+// want_dt is false).
 template <int SEG>
-__global__ void __launch_bounds__(256, 2)
-k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
-           const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+__device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const State& Un,
+                                          const State& U, const Faces& F, const State& Uo,
+                                          double alpha, double beta, double dt, int repair,
+                                          bool want_dt, int strip, int seg) {
   const int lane = threadIdx.x & (MW - 1);
-  const int wave = threadIdx.x / MW;
-  const int j = G.g + (int)blockIdx.x * MJ - MH + lane;
-  const int i0 = G.g + ((int)blockIdx.y * (blockDim.x / MW) + wave) * SEG;
-  if (i0 >= G.g + G.nx) return;  // wave-uniform
+  const int j = G.g + strip * MJ - MH + lane;
+  const int i0 = G.g + seg * SEG;
+  if (i0 >= G.g + G.nx) return 1.0e30;  // wave-uniform
   const int i_end = min(i0 + SEG, G.g + G.nx);
   const size_t ld = (size_t)G.ld;
-  const double dt = *dt_dev;
   const double inv_dx = 1.0 / G.dx;
   const double inv_dy = 1.0 / G.dy;
   const bool col_in = j >= G.g && j < G.g + G.ny;          // interior column
@@ -814,7 +825,7 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
         un[5] = clamp_al(un[5]);
 #pragma unroll
         for (int m = 0; m < 6; ++m) Uo.f[m][c] = un[m];
-        if (dt_acc) local_dt = wmin(local_dt, cell_dt(e, un, G.dx, G.dy));
+        if (want_dt) local_dt = wmin(local_dt, cell_dt(e, un, G.dx, G.dy));
       }
     }
 #pragma unroll
@@ -823,13 +834,26 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
       F2yu[m] = F2yn[m];
     }
   }
-  if (dt_acc) {
-    double v = local_dt;
+  return local_dt;
+}
+
+__device__ __forceinline__ void wave_min_atomic(double v, double* acc) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = wmin(v, __shfl_xor(v, off, 64));
-    if (lane == 0)
-      atomicMin((unsigned long long*)dt_acc, (unsigned long long)__double_as_longlong(v));
-  }
+  for (int off = 32; off > 0; off >>= 1) v = wmin(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & (MW - 1)) == 0)
+    atomicMin((unsigned long long*)acc, (unsigned long long)__double_as_longlong(v));
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(256, 2)
+k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
+           const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+  const int wave = threadIdx.x / MW;
+  const int seg = (int)blockIdx.y * (blockDim.x / MW) + wave;
+  if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
+  double v = lu_wave<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
+                          (int)blockIdx.x, seg);
+  if (dt_acc) wave_min_atomic(v, dt_acc);
 }
 
 // ========================================================================
@@ -949,6 +973,110 @@ __global__ void __launch_bounds__(STHREADS) k_bc(Geo G, State U, double* up, dou
 }
 
 // ========================================================================
+// Batched mode: many independent cases (any grid and EOS, one scheme and
+// precision) advanced by one persistent launch per kernel kind.  The work
+// list is the concatenation of every active case's face tiles (faces) or
+// wave items (limiter); a block/wave finds its case by binary search in the
+// prefix offsets.  Per-case dt, time, end time and a done flag live on the
+// device, so a captured step graph needs no host interaction.
+// ========================================================================
+__device__ __forceinline__ int find_case(const int* __restrict__ off, int n, int t) {
+  int lo = 0, hi = n;  // off[lo] <= t < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(FTHREADS)
+k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off, int ncase,
+              BatchScal sc, int stage, int vf_always, double t1) {
+  __shared__ FaceSmem<TL> S;
+  const int total = tile_off[ncase];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int k = find_case(tile_off, ncase, t);
+    if (sc.done[k]) continue;  // block-uniform
+    const CaseDesc& c = cs[k];
+    const int lt = t - tile_off[k];
+    State U;
+    Faces F;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) U.f[m] = c.u[stage][m];
+#pragma unroll
+    for (int m = 0; m < 28; ++m) F.f[m] = c.f[m];
+    faces_tile<TL, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, F, vf_always, t1, lt / c.tiles_j,
+                                           lt % c.tiles_j, S);
+    __syncthreads();  // the next tile overwrites the LDS
+  }
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(256, 2)
+k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, int ncase,
+           BatchScal sc, int stage, int repair) {
+  const int waves = blockDim.x / MW;
+  const int total = item_off[ncase];
+  const double alpha = stage == 0 ? 0.0 : stage == 1 ? 0.75 : 1.0 / 3.0;
+  const double beta = stage == 0 ? 1.0 : stage == 1 ? 0.25 : 2.0 / 3.0;
+  for (int it = blockIdx.x * waves + threadIdx.x / MW; it < total; it += gridDim.x * waves) {
+    const int k = find_case(item_off, ncase, it);
+    if (sc.done[k]) continue;  // wave-uniform
+    const CaseDesc& c = cs[k];
+    const int li = it - item_off[k];
+    State Un, U, Uo;
+    Faces F;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      Un.f[m] = c.u[0][m];
+      U.f[m] = c.u[stage][m];
+      Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
+    }
+#pragma unroll
+    for (int m = 0; m < 28; ++m) F.f[m] = c.f[m];
+    double v = lu_wave<SEG>(c.geo, c.eos, Un, U, F, Uo, alpha, beta, sc.dt[k], repair, stage == 2,
+                            li % c.strips, li / c.strips);
+    if (stage == 2) wave_min_atomic(v, sc.acc + k);
+  }
+}
+
+// After a step: t += dt; next dt = cfl * min, clipped to the end time
+// (SC:2025-2026); a case that reached its end time is frozen.
+__global__ void k_batch_finalize(const CaseDesc* __restrict__ cs, int ncase, BatchScal sc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncase || sc.done[k]) return;
+  const double t = sc.t[k] + sc.dt[k];
+  sc.t[k] = t;
+  double dt = cs[k].cfl * sc.acc[k];
+  sc.acc[k] = 1.0e30;
+  const double te = sc.tend[k];
+  if (te > 0.0) {
+    if (t >= te - 1.0e-14) {
+      sc.done[k] = 1;
+      dt = 0.0;
+    } else if (t + dt > te) {
+      dt = te - t;
+    }
+  }
+  sc.dt[k] = dt;
+}
+
+// Initial dt of one slot from get_dt's accumulator (clipped to the end time).
+__global__ void k_batch_dt0(double* acc, BatchScal sc, int k, double cfl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double dt = cfl * *acc;
+    *acc = 1.0e30;
+    const double te = sc.tend[k];
+    if (te > 0.0 && sc.t[k] + dt > te) dt = te - sc.t[k];
+    sc.dt[k] = dt;
+    sc.acc[k] = 1.0e30;
+    sc.done[k] = (te > 0.0 && sc.t[k] >= te - 1.0e-14) ? 1 : 0;
+  }
+}
+
+// ========================================================================
 // launch helpers (called by the C-ABI layer)
 // ========================================================================
 typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
@@ -956,9 +1084,10 @@ typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
 template <int BASE, int SHEAR, int FACEPP, typename T>
 static void launch_faces_t(const Geo& G, const Eos& e, const State& U, const Faces& F,
                            int vf_always, double t1, hipStream_t st) {
-  dim3 grid((G.ny + 1 + FTJ - 1) / FTJ, (G.nx + 1 + FTI - 1) / FTI);
-  hipLaunchKernelGGL((k_faces<BASE, SHEAR, FACEPP, T>), grid, dim3(FTHREADS), 0, st, G, e, U, F,
-                     vf_always, t1);
+  using TL = Tile8x32;
+  dim3 grid((G.ny + 1 + TL::TJ - 1) / TL::TJ, (G.nx + 1 + TL::TI - 1) / TL::TI);
+  hipLaunchKernelGGL((k_faces<TL, BASE, SHEAR, FACEPP, T>), grid, dim3(FTHREADS), 0, st, G, e, U,
+                     F, vf_always, t1);
 }
 
 }  // namespace wamp
@@ -1061,4 +1190,57 @@ hipError_t wamp_launch_bc(const Geo* G, const State* U, double* up, double* vp, 
                      vp, pp);
   return hipGetLastError();
 }
+
+hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int ncase,
+                                   const BatchScal* sc, int stage, int base, int shear,
+                                   int face_pp, int vf_always, double t1, int precision, int tile,
+                                   int blocks, hipStream_t st) {
+#define WAMP_FB(TL, B, S, P, T)                                                              \
+  hipLaunchKernelGGL((k_faces_batch<TL, B, S, P, T>), dim3(blocks), dim3(FTHREADS), 0, st, cs, \
+                     tile_off, ncase, *sc, stage, vf_always, t1)
+#define WAMP_FBS(B, S, P)                                                          \
+  if (base == B && shear == S && face_pp == P) {                                  \
+    if (tile == 0 && precision == 0) WAMP_FB(Tile8x32, B, S, P, double);           \
+    else if (tile == 0) WAMP_FB(Tile8x32, B, S, P, float);                         \
+    else if (precision == 0) WAMP_FB(Tile16x16, B, S, P, double);                  \
+    else WAMP_FB(Tile16x16, B, S, P, float);                                       \
+    return hipGetLastError();                                                      \
+  }
+  // the three named schemes: SC (0,1,0), M10 (0,0,1), MUSCL/MP5 + central6 (1,2,0)
+  WAMP_FBS(0, 1, 0) WAMP_FBS(0, 0, 1) WAMP_FBS(1, 2, 0)
+#undef WAMP_FBS
+#undef WAMP_FB
+  return hipErrorInvalidValue;
+}
+
+hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int ncase,
+                                const BatchScal* sc, int stage, int repair, int seg, int blocks,
+                                hipStream_t st) {
+#define WAMP_LB(S)                                                                            \
+  if (seg == S) {                                                                             \
+    hipLaunchKernelGGL(k_lu_batch<S>, dim3(blocks), dim3(256), 0, st, cs, item_off, ncase, *sc, \
+                       stage, repair);                                                        \
+    return hipGetLastError();                                                                 \
+  }
+  WAMP_LB(8) WAMP_LB(16) WAMP_LB(32)
+#undef WAMP_LB
+  return hipErrorInvalidValue;
+}
+
+hipError_t wamp_launch_batch_finalize(const CaseDesc* cs, int ncase, const BatchScal* sc,
+                                      hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_finalize, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase, *sc);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_batch_dt0(const Geo* G, const Eos* e, const State* U, double* acc,
+                                 const BatchScal* sc, int k, double cfl, hipStream_t st) {
+  const int n = G->nx * G->ny;
+  hipLaunchKernelGGL(k_dt, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G, *e, *U,
+                     acc);
+  hipLaunchKernelGGL(k_batch_dt0, dim3(1), dim3(64), 0, st, acc, *sc, k, cfl);
+  return hipGetLastError();
+}
+
+size_t wamp_case_desc_size(void) { return sizeof(CaseDesc); }
 }

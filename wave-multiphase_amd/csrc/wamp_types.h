@@ -1,0 +1,38 @@
+// wamp_types.h -- argument structures shared by the kernels and the C-ABI layer.
+#pragma once
+
+namespace wamp {
+
+struct Geo {  // one case's grid: interior nx x ny, g ghost layers, row stride ld
+  int nx, ny, g, ld;
+  double dx, dy;
+  int bc[4];
+};
+struct State {  // the six conserved planes [a1 rho1, a2 rho2, rho u, rho v, E, a1]
+  double* f[6];
+};
+struct Faces {  // fx0..5, srcx, gy0..5, srcy, fxl0..5, srcxl, gyl0..5, srcyl
+  double* f[28];
+};
+
+// One case of a batch (device-resident table).
+struct CaseDesc {
+  Geo geo;
+  Eos eos;
+  double* u[3][6];  // [0] state (U^n, overwritten by stage 3), [1] U^1, [2] U^2
+  double* f[28];    // face planes
+  int tiles_j, tiles;        // face tiles along j and in total
+  int strips, items;         // limiter work items: strips x segments
+  double cfl;
+};
+
+// Per-slot device scalars of a batch.
+struct BatchScal {
+  double* dt;
+  double* acc;
+  double* t;
+  double* tend;  // <= 0: no end time
+  int* done;
+};
+
+}  // namespace wamp

@@ -54,6 +54,20 @@ SIGNATURES = [
                                     C.c_double, _VP, _VP, _VP]),
     ("wamp_get_dt", C.c_int, [_VP, PtrArr6, _VP, _VP]),
     ("wamp_step", C.c_int, [_VP, PtrArr6, _VP, _VP, _VP]),
+    ("wamp_batch_create", C.c_int, [C.POINTER(WampScheme), C.c_int32, C.c_int32, C.POINTER(_VP)]),
+    ("wamp_batch_destroy", C.c_int, [_VP]),
+    ("wamp_batch_set_case", C.c_int, [_VP, C.c_int32, C.POINTER(WampGrid), C.POINTER(WampEos),
+                                      PtrArr6, C.c_double, C.c_double]),
+    ("wamp_batch_clear_case", C.c_int, [_VP, C.c_int32]),
+    ("wamp_batch_set_tuning", C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    ("wamp_batch_commit", C.c_int, [_VP, _VP]),
+    ("wamp_batch_prepare", C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, _VP]),
+    ("wamp_batch_set_done", C.c_int, [_VP, C.c_int32, C.c_int32, _VP]),
+    ("wamp_batch_graph", C.c_int, [_VP, C.c_int32]),
+    ("wamp_batch_step", C.c_int, [_VP, _VP]),
+    ("wamp_batch_status", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int32), _VP]),
+    ("wamp_batch_work", C.c_int64, [_VP, C.c_int32]),
     ("wamp_timing_enable", C.c_int, [_VP, C.c_int32]),
     ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
 ]
