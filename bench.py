@@ -111,6 +111,85 @@ def mixed_leg(cfg, U0, warmup: int, steps: int):
     }
 
 
+def batched_leg(n_cases: int, steps: int, seed: int = 4):
+    """64 seeded uneven shock-droplet cases: plain FP64 one case at a time versus
+    one batch in mixed precision (autotuned, step graph-captured).  Both run
+    `steps` steps per case; cases/hour is normalised to 500-step cases."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from wamp import accuracy, cases
+    from wamp.batch import Batch
+    from wamp.config import PREC_MIXED, SCHEME_MP5
+    from wamp.solver import Solver
+
+    rng = np.random.default_rng(seed)
+    work = [cases.batch_case(rng, scheme=SCHEME_MP5) for _ in range(n_cases)]
+    cells = sum(c.grid.nx * c.grid.ny for c, _ in work)
+    # plain FP64, one case after another (each case's state resident on the GPU)
+    plain_states = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for cfg, U0 in work:
+        s = Solver(cfg)
+        s.set_state(U0)
+        s.prepare()
+        s.compute_dt()
+        for _ in range(steps):
+            s.step_device()
+        if len(plain_states) < 4:
+            plain_states.append(s.state())
+        s.close()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    # batched mixed precision
+    b = Batch(SCHEME_MP5, PREC_MIXED, max_cases=n_cases)
+    for k, (cfg, U0) in enumerate(work):
+        b.set_case(k, dataclasses.replace(cfg, precision=PREC_MIXED), U0)
+    b.commit()
+    b.prepare()
+    tuning = b.autotune(steps=2)
+    gsteps = 10 if steps % 10 == 0 else 1
+    b.capture(gsteps)
+    snap = {k: v.clone() for k, v in b.states.items()}
+    b.step()  # first replay (warm-up), then back to the initial states
+    for k, v in snap.items():
+        b.states[k].copy_(v)
+    b.prepare()
+    del snap
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        done += b.step()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    spot = []
+    for k in range(min(4, n_cases)):
+        cfg = work[k][0]
+        spot.append(accuracy.compare(cfg, b.state(k), plain_states[k]))
+    b.close()
+    scale = 500.0 / steps
+    return {
+        "workload": f"{n_cases} shock-droplet cases, grids from "
+                    f"{sorted(set((c.grid.nx, c.grid.ny) for c, _ in work))}, seed {seed}",
+        "cells": cells,
+        "steps_per_case": steps,
+        "plain_fp64_cases_per_hour": round(n_cases / (t_plain * scale) * 3600.0, 2),
+        "batched_mixed_cases_per_hour": round(n_cases / (t_batch * scale) * 3600.0, 2),
+        "speedup": round(t_plain / t_batch, 3),
+        "plain_fp64_gcell_updates_per_s": round(cells * steps / t_plain / 1e9, 4),
+        "batched_mixed_gcell_updates_per_s": round(cells * steps / t_batch / 1e9, 4),
+        "tuning": list(tuning.as_tuple()),
+        "graph_steps": gsteps,
+        "spot_check_vs_fp64": {
+            k: float(max(e[k] for e in spot)) for k in ("rho", "p", "alpha1", "centroid_shift_cells")
+        },
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +201,9 @@ def main(argv=None):
                     help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the FP32-mixed leg on the same case (0 = skip)")
+    ap.add_argument("--batch-cases", type=int, default=64,
+                    help="cases of the batched leg (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=100, help="steps per case in the batched leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
                          "of known size, to scale hardware traffic counters)")
@@ -229,11 +311,13 @@ def main(argv=None):
     }
     if args.mixed_steps > 0:
         out["mixed"] = mixed_leg(cfg, U0, args.warmup, args.mixed_steps)
+    s.close()
+    if args.batch_cases > 0:
+        out["batched"] = batched_leg(args.batch_cases, args.batch_steps)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, U0, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
-    s.close()
     print(json.dumps(out), flush=True)
     return 0
 

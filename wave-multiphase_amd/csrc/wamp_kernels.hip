@@ -186,7 +186,20 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, i
     if (phR[1] < T(0) || isnan_(phR[1])) phR[1] = phfR[1];
     if (phR[5] < T(0) || phR[5] > T(1)) phR[5] = phfR[5];
     if (isnan_(phR[4])) phR[4] = phfR[4];
-  } else {  // M10:1007-1021, thresholds always in FP64
+  } else if constexpr (sizeof(T) == sizeof(double)) {  // M10:1007-1021
+    pp_limit_state(e, phL, phfL);
+    pp_limit_state(e, phR, phfR);
+    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
+    if (rho_ct2_sc(e, phL) < PP_CTILDE || isnan_(phL[4])) phL[4] = phfL[4];
+    floor_pressure(e, phL);
+    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
+    if (rho_ct2_sc(e, phR) < PP_CTILDE || isnan_(phR[4])) phR[4] = phfR[4];
+    floor_pressure(e, phR);
+  } else {  // the same on FP64 copies: the thresholds are always applied in FP64
     double dL[6], dR[6], dfL[6], dfR[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
@@ -341,7 +354,7 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(FTHREADS)
+__global__ void __launch_bounds__(FTHREADS, BASE == 0 ? 3 : 2)
 k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
   __shared__ FaceSmem<TL> S;
   faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, (int)blockIdx.y,
@@ -991,7 +1004,7 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, int n, int
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(FTHREADS)
+__global__ void __launch_bounds__(FTHREADS, BASE == 0 ? 3 : 2)
 k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off, int ncase,
               BatchScal sc, int stage, int vf_always, double t1) {
   __shared__ FaceSmem<TL> S;
