@@ -92,14 +92,16 @@ int wamp_prepare(wamp_ctx *ctx, double *const U[6], void *stream);
    fx_kernel, gy_kernel, fx_low_kernel, gy_low_kernel).  F receives 28
    arrays of the state's shape in the reference's order: fx0..fx5, srcx,
    gy0..gy5, srcy, fxl0..fxl5, srcxl, gyl0..gyl5, srcyl.  x-face values sit
-   at [i, j] for face i+1/2, i in [g-1, g+nx-1]; y-faces likewise.  The
-   boundary fill is applied on the fly; U's ghost layers are not read. */
+   at [i, j] for face i+1/2, i in [g-1, g+nx-1]; y-faces likewise.  The 28
+   planes must be equally spaced in one allocation (e.g. a (28, nx+2g, ld)
+   array): F[m] = F[0] + m * (F[1] - F[0]).  The boundary fill is applied on
+   the fly; U's ghost layers are not read. */
 int wamp_compute_fluxes(wamp_ctx *ctx, double *const U[6], double *const F[28], void *stream);
 
 /* limit_fluxes + rk3_update + enforce_admissible + update_prims,
    SC:1949-1984, 1907-1917: the three Wong stages, the SSP-RK3 combine
    Uout = alpha*Un + beta*(U - dt*R(F)), repair and the a1 clamp, on the
-   interior.  Uout must not alias U (halo cells of U are read by
+   interior.  F as in wamp_compute_fluxes.  Uout must not alias U (halo cells of U are read by
    neighbouring tiles); it may alias Un.  dt is read from device memory.
    When dt_acc is non-NULL the CFL minimum of the new state is folded into
    *dt_acc (atomic min, timestep_kernel SC:1686-1705). */

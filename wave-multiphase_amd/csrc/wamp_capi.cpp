@@ -167,7 +167,8 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   if (rc) { delete c; return rc; }
   c->bytes = (int64_t)bytes;
   double* p = c->block;
-  for (int m = 0; m < 28; ++m, p += plane) c->faces.f[m] = p;
+  c->faces = Faces{p, (long long)plane};
+  p += 28 * plane;
   for (int m = 0; m < 6; ++m, p += plane) c->u1.f[m] = p;
   for (int m = 0; m < 6; ++m, p += plane) c->u2.f[m] = p;
   c->dt_acc = p;
@@ -241,6 +242,21 @@ int wamp_prepare(wamp_ctx* c, double* const U[6], void* stream) {
                    "prepare");
 }
 
+static int faces_of(const wamp_ctx* c, double* const F[28], Faces& f) {
+  if (!F) return fail(-2, "F is NULL");
+  for (int m = 0; m < 28; ++m)
+    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
+  const long long st = (long long)(F[1] - F[0]);
+  const long long plane = (long long)(c->geo.nx + 2 * c->geo.g) * c->geo.ld;
+  if (st < plane && -st < plane)
+    return fail(-2, "face planes overlap (spacing %lld < plane %lld elements)", st, plane);
+  for (int m = 2; m < 28; ++m)
+    if ((long long)(F[m] - F[0]) != (long long)m * st)
+      return fail(-2, "the 28 face planes must be equally spaced in one allocation (F[%d])", m);
+  f = Faces{F[0], st};
+  return 0;
+}
+
 static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& f,
                      const State& uo, double alpha, double beta, const double* dt_dev,
                      double* dt_acc, hipStream_t st) {
@@ -262,12 +278,8 @@ int wamp_compute_fluxes(wamp_ctx* c, double* const U[6], double* const F[28], vo
   if (int rc = check_ctx(c)) return rc;
   State s;
   if (int rc = state_of(U, s, "U")) return rc;
-  if (!F) return fail(-2, "F is NULL");
   Faces f;
-  for (int m = 0; m < 28; ++m) {
-    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
-    f.f[m] = F[m];
-  }
+  if (int rc = faces_of(c, F, f)) return rc;
   return launch_faces(c, s, f, (hipStream_t)stream);
 }
 
@@ -281,12 +293,9 @@ int wamp_limit_update(wamp_ctx* c, double* const Un[6], double* const U[6], doub
   if (int rc = state_of(Uout, so, "Uout")) return rc;
   for (int m = 0; m < 6; ++m)
     if (so.f[m] == s.f[m]) return fail(-2, "Uout[%d] aliases U[%d]", m, m);
-  if (!F || !dt_dev) return fail(-2, "F and dt_dev must be non-NULL");
+  if (!dt_dev) return fail(-2, "dt_dev must be non-NULL");
   Faces f;
-  for (int m = 0; m < 28; ++m) {
-    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
-    f.f[m] = F[m];
-  }
+  if (int rc = faces_of(c, F, f)) return rc;
   return launch_lu(c, sn, s, f, so, alpha, beta, dt_dev, dt_acc, (hipStream_t)stream);
 }
 
@@ -527,7 +536,8 @@ int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, cons
               {grid->bc[0], grid->bc[1], grid->bc[2], grid->bc[3]}};
   d.eos = Eos{eos->gamma1, eos->pinf1, eos->gamma2, eos->pinf2};
   for (int m = 0; m < 6; ++m) d.u[0][m] = U[m];
-  for (int m = 0; m < 28; ++m, p += plane) d.f[m] = p;
+  d.f = Faces{p, (long long)plane};
+  p += 28 * plane;
   for (int m = 0; m < 6; ++m, p += plane) d.u[1][m] = p;
   for (int m = 0; m < 6; ++m, p += plane) d.u[2][m] = p;
   d.cfl = b->scheme.cfl;
@@ -638,7 +648,7 @@ int wamp_batch_set_done(wamp_batch* b, int32_t slot, int32_t done, void* stream)
 static int batch_enqueue_step(wamp_batch* b, hipStream_t st) {
   const int n = b->max_cases;
   const int fb = std::max(1, std::min(b->total_tiles, b->faces_bpcu * b->cu_count));
-  const int lb = std::max(1, std::min((b->total_items + 3) / 4, b->lu_bpcu * b->cu_count));
+  const int lb = std::max(1, (b->total_items + 3) / 4);  // one wave per limiter item
   for (int s = 0; s < 3; ++s) {
     if (int rc = hip_check(wamp_launch_faces_batch(b->desc_dev, b->off_dev, n, &b->sc, s,
                                                    b->scheme.base, b->scheme.shear,

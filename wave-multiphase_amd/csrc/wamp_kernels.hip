@@ -332,10 +332,10 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
     }
     face_high<TL, 0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F.f[m][o] = out[m];
+    for (int m = 0; m < 7; ++m) F[m][o] = out[m];
     face_low<TL, 0, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F.f[14 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) F[14 + m][o] = out[m];
   }
   if (i >= G.g && i <= G.g + G.nx - 1 && j >= G.g - 1 && j <= G.g + G.ny - 1) {
     double ci[4], sg[4], out[7];
@@ -346,10 +346,10 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
     }
     face_high<TL, 1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F.f[7 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) F[7 + m][o] = out[m];
     face_low<TL, 1, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F.f[21 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) F[21 + m][o] = out[m];
   }
 }
 
@@ -396,7 +396,7 @@ __device__ __forceinline__ double face_theta(double ta, double tb) {
 __device__ __forceinline__ void residual(const Faces& F, int m, size_t c, size_t ld,
                                          const CellFaces& cf, int nb, double inv_dx,
                                          double inv_dy, double a, double& rh, double& rl) {
-  const double *fx = F.f[m], *gy = F.f[7 + m], *fxl = F.f[14 + m], *gyl = F.f[21 + m];
+  const double *fx = F[m], *gy = F[7 + m], *fxl = F[14 + m], *gyl = F[21 + m];
   const size_t cx = c - ld, cy = c - 1;
   double Lxp = fxl[c], Lxm = fxl[cx], Lyp = gyl[c], Lym = gyl[cy];
   double Hxp = blend(fx[c], Lxp, cf.t[1], nb);
@@ -406,7 +406,7 @@ __device__ __forceinline__ void residual(const Faces& F, int m, size_t c, size_t
   rh = (Hxp - Hxm) * inv_dx + (Hyp - Hym) * inv_dy;
   rl = (Lxp - Lxm) * inv_dx + (Lyp - Lym) * inv_dy;
   if (m == 5) {
-    const double *sx = F.f[6], *sy = F.f[13], *sxl = F.f[20], *syl = F.f[27];
+    const double *sx = F[6], *sy = F[13], *sxl = F[20], *syl = F[27];
     double sLxp = sxl[c], sLxm = sxl[cx], sLyp = syl[c], sLym = syl[cy];
     double sHxp = blend(sx[c], sLxp, cf.t[1], nb);
     double sHxm = blend(sx[cx], sLxm, cf.t[0], nb);
@@ -628,7 +628,7 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   auto row_in = [&](int r) { return r >= G.g && r < G.g + G.nx; };
   auto xrow_in = [&](int r) { return r >= G.g - 1 && r < G.g + G.nx; };  // x-face r+1/2 exists
   auto at = [&](int r) { return (size_t)r * ld + (size_t)j; };
-  // face component m: high F.f[base + m], low F.f[base + 14 + m]; base 0 = x, 7 = y
+  // face component m: high F[base + m], low F[base + 14 + m]; base 0 = x, 7 = y
 
   double x01[2][4] = {};   // x-face rows f-1, f: H0 H1 L0 L1
   double x56[2][4] = {};   // x-face rows f-2, f-1: H5 H6 L5 L6
@@ -647,20 +647,20 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const size_t c = at(f);
 #pragma unroll
       for (int q = 0; q < 4; ++q) x01[0][q] = x01[1][q];
-      x01[1][0] = ok ? F.f[0][c] : 0.0;
-      x01[1][1] = ok ? F.f[1][c] : 0.0;
-      x01[1][2] = ok ? F.f[14][c] : 0.0;
-      x01[1][3] = ok ? F.f[15][c] : 0.0;
+      x01[1][0] = ok ? F[0][c] : 0.0;
+      x01[1][1] = ok ? F[1][c] : 0.0;
+      x01[1][2] = ok ? F[14][c] : 0.0;
+      x01[1][3] = ok ? F[15][c] : 0.0;
     }
     {
       const bool ok = xrow_in(f - 1) && col_in;
       const size_t c = at(f - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) x56[0][q] = x56[1][q];
-      x56[1][0] = ok ? F.f[5][c] : 0.0;
-      x56[1][1] = ok ? F.f[6][c] : 0.0;
-      x56[1][2] = ok ? F.f[19][c] : 0.0;
-      x56[1][3] = ok ? F.f[20][c] : 0.0;
+      x56[1][0] = ok ? F[5][c] : 0.0;
+      x56[1][1] = ok ? F[6][c] : 0.0;
+      x56[1][2] = ok ? F[19][c] : 0.0;
+      x56[1][3] = ok ? F[20][c] : 0.0;
     }
     {
       const bool ok = xrow_in(f - 2) && col_in;
@@ -669,8 +669,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       for (int m = 0; m < 14; ++m) xa[0][m] = xa[1][m];
 #pragma unroll
       for (int m = 0; m < 7; ++m) {
-        xa[1][m] = ok ? F.f[m][c] : 0.0;
-        xa[1][7 + m] = ok ? F.f[14 + m][c] : 0.0;
+        xa[1][m] = ok ? F[m][c] : 0.0;
+        xa[1][7 + m] = ok ? F[14 + m][c] : 0.0;
       }
     }
 
@@ -684,8 +684,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
           // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
-          double yuh = F.f[7 + m][c], ydh = F.f[7 + m][c - 1];
-          double yul = F.f[21 + m][c], ydl = F.f[21 + m][c - 1];
+          double yuh = F[7 + m][c], ydh = F[7 + m][c - 1];
+          double yul = F[21 + m][c], ydl = F[21 + m][c - 1];
           double rh = (x01[1][m] - x01[0][m]) * inv_dx + (yuh - ydh) * inv_dy;
           double rl = (x01[1][2 + m] - x01[0][2 + m]) * inv_dx + (yul - ydl) * inv_dy;
           double u = U.f[m][c];
@@ -711,10 +711,10 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       tf[2] = face_theta(lane_up(t0[3]), t0[3]);
       tf[3] = face_theta(t0[3], lane_down(t0[3]));
       double yu[4];
-      yu[0] = yok ? F.f[12][c] : 0.0;
-      yu[1] = yok ? F.f[13][c] : 0.0;
-      yu[2] = yok ? F.f[26][c] : 0.0;
-      yu[3] = yok ? F.f[27][c] : 0.0;
+      yu[0] = yok ? F[12][c] : 0.0;
+      yu[1] = yok ? F[13][c] : 0.0;
+      yu[2] = yok ? F[26][c] : 0.0;
+      yu[3] = yok ? F[27][c] : 0.0;
       double yd[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) yd[q] = lane_up(yu[q]);
@@ -752,8 +752,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const bool yok = row_in(r) && yface_in;
 #pragma unroll
       for (int m = 0; m < 7; ++m) {
-        yn[m] = yok ? F.f[7 + m][c] : 0.0;
-        yn[7 + m] = yok ? F.f[21 + m][c] : 0.0;
+        yn[m] = yok ? F[7 + m][c] : 0.0;
+        yn[7 + m] = yok ? F[21 + m][c] : 0.0;
       }
       double tf[4][2];
       tf[0][0] = face_theta(t0[1], t0[2]);
@@ -861,7 +861,7 @@ template <int SEG>
 __global__ void __launch_bounds__(256, 2)
 k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
            const double* __restrict__ dt_dev, double* dt_acc, int repair) {
-  const int wave = threadIdx.x / MW;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
   const int seg = (int)blockIdx.y * (blockDim.x / MW) + wave;
   if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
   double v = lu_wave<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
@@ -1015,44 +1015,41 @@ k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off,
     const CaseDesc& c = cs[k];
     const int lt = t - tile_off[k];
     State U;
-    Faces F;
 #pragma unroll
     for (int m = 0; m < 6; ++m) U.f[m] = c.u[stage][m];
-#pragma unroll
-    for (int m = 0; m < 28; ++m) F.f[m] = c.f[m];
+    const Faces F = c.f;
     faces_tile<TL, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, F, vf_always, t1, lt / c.tiles_j,
                                            lt % c.tiles_j, S);
     __syncthreads();  // the next tile overwrites the LDS
   }
 }
 
+// One wave per work item (case, strip, segment) of the concatenated list.
 template <int SEG>
 __global__ void __launch_bounds__(256, 2)
 k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, int ncase,
            BatchScal sc, int stage, int repair) {
-  const int waves = blockDim.x / MW;
-  const int total = item_off[ncase];
+  // the wave index is uniform across the wave: keep it (and everything derived
+  // from it: case, descriptor, rows) in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);
+  const int it = blockIdx.x * (blockDim.x / MW) + wave;
+  if (it >= item_off[ncase]) return;
+  const int k = find_case(item_off, ncase, it);
+  if (sc.done[k]) return;
+  const CaseDesc& c = cs[k];
+  const int li = it - item_off[k];
   const double alpha = stage == 0 ? 0.0 : stage == 1 ? 0.75 : 1.0 / 3.0;
   const double beta = stage == 0 ? 1.0 : stage == 1 ? 0.25 : 2.0 / 3.0;
-  for (int it = blockIdx.x * waves + threadIdx.x / MW; it < total; it += gridDim.x * waves) {
-    const int k = find_case(item_off, ncase, it);
-    if (sc.done[k]) continue;  // wave-uniform
-    const CaseDesc& c = cs[k];
-    const int li = it - item_off[k];
-    State Un, U, Uo;
-    Faces F;
+  State Un, U, Uo;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      Un.f[m] = c.u[0][m];
-      U.f[m] = c.u[stage][m];
-      Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
-    }
-#pragma unroll
-    for (int m = 0; m < 28; ++m) F.f[m] = c.f[m];
-    double v = lu_wave<SEG>(c.geo, c.eos, Un, U, F, Uo, alpha, beta, sc.dt[k], repair, stage == 2,
-                            li % c.strips, li / c.strips);
-    if (stage == 2) wave_min_atomic(v, sc.acc + k);
+  for (int m = 0; m < 6; ++m) {
+    Un.f[m] = c.u[0][m];
+    U.f[m] = c.u[stage][m];
+    Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
   }
+  double v = lu_wave<SEG>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair, stage == 2,
+                          li % c.strips, li / c.strips);
+  if (stage == 2) wave_min_atomic(v, sc.acc + k);
 }
 
 // After a step: t += dt; next dt = cfl * min, clipped to the end time

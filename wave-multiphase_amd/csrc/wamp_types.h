@@ -11,8 +11,12 @@ struct Geo {  // one case's grid: interior nx x ny, g ghost layers, row stride l
 struct State {  // the six conserved planes [a1 rho1, a2 rho2, rho u, rho v, E, a1]
   double* f[6];
 };
-struct Faces {  // fx0..5, srcx, gy0..5, srcy, fxl0..5, srcxl, gyl0..5, srcyl
-  double* f[28];
+// The 28 face planes fx0..5, srcx, gy0..5, srcy, fxl0..5, srcxl, gyl0..5,
+// srcyl, equally spaced in one allocation: plane m starts at b + m * s.
+struct Faces {
+  double* b;
+  long long s;
+  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }
 };
 
 // One case of a batch (device-resident table).
@@ -20,7 +24,7 @@ struct CaseDesc {
   Geo geo;
   Eos eos;
   double* u[3][6];  // [0] state (U^n, overwritten by stage 3), [1] U^1, [2] U^2
-  double* f[28];    // face planes
+  Faces f;          // face planes
   int tiles_j, tiles;        // face tiles along j and in total
   int strips, items;         // limiter work items: strips x segments
   double cfl;
