@@ -354,7 +354,7 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(FTHREADS, BASE == 0 ? 3 : 2)
+__global__ void __launch_bounds__(FTHREADS, (BASE == 0 || sizeof(T) == 4) ? 3 : 2)
 k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
   __shared__ FaceSmem<TL> S;
   faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, (int)blockIdx.y,
@@ -1004,7 +1004,7 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, int n, int
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(FTHREADS, BASE == 0 ? 3 : 2)
+__global__ void __launch_bounds__(FTHREADS, (BASE == 0 || sizeof(T) == 4) ? 3 : 2)
 k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off, int ncase,
               BatchScal sc, int stage, int vf_always, double t1) {
   __shared__ FaceSmem<TL> S;
