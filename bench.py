@@ -190,6 +190,65 @@ def batched_leg(n_cases: int, steps: int, seed: int = 4):
     }
 
 
+def serving_leg(n: int, seed: int = 5, load: float = 0.7):
+    """Continuous batching of `n` requests (log-uniform grids 128..2048 wide; half
+    shock-droplet cases to t=0.05, half triple points to t=0.5).
+
+    plain      the plain FP64 solver serving the queue one request at a time
+    saturated  the batched mixed server with every request queued at t=0
+               (its cases/hour is the server's capacity)
+    poisson    the batched mixed server under Poisson arrivals at `load` x capacity
+    Results of both server runs are checked bitwise against each other (same
+    cases, different mixes) and against the plain FP64 results (relative L1)."""
+    import numpy as np
+
+    from wamp import accuracy, serve
+    from wamp.config import PREC_FP64
+
+    reqs = serve.poisson_workload(n, 1.0, seed)
+    warm = [r for r in reqs if r.cfg.grid.nx <= 192][:2]
+    serve.serve_plain(serve.with_arrivals(warm, None, precision=PREC_FP64))
+    serve.serve_batched(serve.with_arrivals(warm, None))
+    plain_states = {}
+    _, plain = serve.serve_plain(serve.with_arrivals(reqs, None, precision=PREC_FP64),
+                                 keep=lambda rid, U: plain_states.__setitem__(rid, U), log=_progress)
+    _progress(f"serving: plain fp64 {n} requests {plain['elapsed_s']:.1f} s")
+    res_sat, sat = serve.serve_batched(serve.with_arrivals(reqs, None), log=_progress)
+    _progress(f"serving: batched saturated {sat['elapsed_s']:.1f} s")
+    rate = load * sat["completed"] / sat["elapsed_s"]
+    res_poi, poi = serve.serve_batched(serve.with_arrivals(reqs, rate), log=_progress)
+    _progress(f"serving: batched poisson at {rate:.2f}/s {poi['elapsed_s']:.1f} s")
+    by_rid = {r.rid: r for r in reqs}
+    poi_states = {r.rid: r.state for r in res_poi}
+    same, errs = True, {"droplets": [], "triple_point": []}
+    for r in res_sat:
+        cfg = by_rid[r.rid].cfg
+        si, sj = cfg.grid.interior()
+        same &= bool(np.array_equal(r.state[:, si, sj], poi_states[r.rid][:, si, sj]))
+        kind = "triple_point" if cfg.name.startswith("triple") else "droplets"
+        errs[kind].append(accuracy.compare(cfg, r.state, plain_states[r.rid]))
+    keys = ("rho", "p", "alpha1", "centroid_shift_cells")
+    cells = sum(r.cfg.grid.nx * r.cfg.grid.ny for r in reqs)
+    return {
+        "workload": f"{n} requests, seed {seed}: NX log-uniform in [128, 2048], NY = NX/2 "
+                    f"(128..1024); droplets (MP5, t_end {serve.T_END_DROPLET}) and triple "
+                    f"points (MP5, t_end {serve.T_END_TRIPLE})",
+        "mean_cells": round(cells / n),
+        "plain_fp64": {k: round(v, 4) for k, v in plain.items()},
+        "batched_mixed_saturated": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.items()},
+        "batched_mixed_poisson": dict({k: (round(v, 4) if isinstance(v, float) else v) for k, v in poi.items()},
+                                      rate_per_s=round(rate, 4), load=load),
+        "speedup_cases_per_hour": round(sat["cases_per_hour"] / plain["cases_per_hour"], 3),
+        "mix_invariant_bitwise": same,
+        "max_rel_l1_vs_fp64": {kind: {k: float(max(e[k] for e in v)) for k in keys}
+                               for kind, v in errs.items() if v},
+    }
+
+
+def _progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +263,8 @@ def main(argv=None):
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases of the batched leg (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=100, help="steps per case in the batched leg")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the serving leg (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
                          "of known size, to scale hardware traffic counters)")
@@ -312,8 +373,12 @@ def main(argv=None):
     if args.mixed_steps > 0:
         out["mixed"] = mixed_leg(cfg, U0, args.warmup, args.mixed_steps)
     s.close()
+    _progress(f"headline {out['value']:.4f} Gcell-updates/s")
     if args.batch_cases > 0:
         out["batched"] = batched_leg(args.batch_cases, args.batch_steps)
+        _progress(f"batched speedup {out['batched']['speedup']}")
+    if args.serve_requests > 0:
+        out["serving"] = serving_leg(args.serve_requests)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, U0, args.cpu_steps)
     else:

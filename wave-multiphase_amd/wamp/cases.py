@@ -63,16 +63,21 @@ def shock_droplets(grid: Grid, eos: EOS, mach: float, shock_x: float, droplets,
     solid droplet.  The interface is a tanh profile over `smear_cells` cells,
     alpha clipped to [1e-6, 1-1e-6].
     """
-    x = grid.x_centres()[:, None]
-    y = grid.y_centres()[None, :]
+    xs = grid.x_centres()
+    ys = grid.y_centres()
+    x = xs[:, None]
     w = smear_cells * grid.dx
     water = np.zeros((grid.nxg, grid.nyg))
     for (xc, yc, r, rc) in droplets:
-        d = np.sqrt((x - xc) ** 2 + (y - yc) ** 2)
+        # outside r + 12 w the profile is below 1e-10, i.e. lost in the alpha clip
+        reach = r + 12.0 * w
+        i0, i1 = np.searchsorted(xs, xc - reach), np.searchsorted(xs, xc + reach)
+        j0, j1 = np.searchsorted(ys, yc - reach), np.searchsorted(ys, yc + reach)
+        d = np.sqrt((xs[i0:i1, None] - xc) ** 2 + (ys[None, j0:j1] - yc) ** 2)
         a = _smeared(d, r, w)
         if rc > 0.0:
             a = a * (1.0 - _smeared(d, rc, w))
-        water = np.maximum(water, a)
+        np.maximum(water[i0:i1, j0:j1], a, out=water[i0:i1, j0:j1])
     a1 = np.clip(water, ALPHA_CLIP, 1.0 - ALPHA_CLIP)
     rho_s, p_s, u_s = rankine_hugoniot(mach, eos.gamma2, rho_air, p0)
     post = np.broadcast_to(x < shock_x, a1.shape)

@@ -1,0 +1,269 @@
+"""Serving loop: continuous batching of solve requests on one GPU.
+
+Requests (a case, its initial state, an end time, an arrival time) queue up
+as they arrive.  Between device rounds the loop
+  * retires every case that reached its end time: the result is copied to
+    the host, its latency (result on host - arrival) recorded, its slot and
+    device memory released;
+  * admits queued requests while a slot is free and the request's device
+    memory (state + scratch, 46 FP64 planes of its padded grid) fits in the
+    HBM budget -- the new cases join the running batch;
+  * advances every resident case by one round of `round_steps` steps (one
+    hipGraph replay).
+Cases therefore enter and leave while others are running: their work
+overlaps on the device in the same persistent launches.  A case's result
+does not depend on what else is in the batch (tests/test_gpu_serve.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import cases
+from .batch import Batch, Tuning
+from .config import PREC_MIXED, SCHEME_MP5, CaseConfig
+
+PLANES_PER_CASE = 46  # 6 state planes + 28 face planes + 2 x 6 stage planes
+
+
+@dataclass(order=True)
+class Request:
+    arrival: float
+    rid: int = field(compare=False)
+    cfg: CaseConfig = field(compare=False)
+    U0: np.ndarray = field(compare=False, repr=False)
+    t_end: float = field(compare=False)
+
+    @property
+    def nbytes(self) -> int:
+        g = self.cfg.grid
+        return PLANES_PER_CASE * g.nxg * g.nyg * 8
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    admitted_round: int
+    finished_round: int
+    state: np.ndarray
+    t: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class Server:
+    def __init__(self, scheme=SCHEME_MP5, precision: int = PREC_MIXED, max_slots: int = 64,
+                 hbm_budget: float = 64e9, round_steps: int = 10, tuning: Tuning | None = None,
+                 clock=time.perf_counter, sleep=time.sleep):
+        self.batch = Batch(scheme, precision, max_slots)
+        self.scheme, self.precision = scheme, precision
+        self.max_slots = max_slots
+        self.budget = float(hbm_budget)
+        self.used = 0.0
+        self.round_steps = round_steps
+        self.clock = clock
+        self.sleep = sleep
+        self.peak_used = 0.0
+        self.peak_resident = 0
+        self.free = list(range(max_slots))
+        self.slot_req: dict[int, tuple[Request, float, int]] = {}
+        self.queue: list[Request] = []
+        self.results: list[Result] = []
+        self.rounds = 0
+        if tuning:
+            self.batch.set_tuning(tuning)
+
+    def submit(self, req: Request):
+        if req.nbytes > self.budget:
+            raise ValueError(f"request {req.rid} needs {req.nbytes / 1e9:.2f} GB > budget")
+        heapq.heappush(self.queue, req)
+
+    # one admission / retirement pass; returns True if the batch changed
+    def _retire(self) -> bool:
+        if not self.slot_req:
+            return False
+        t, _, done = self.batch.status()
+        changed = False
+        now = self.clock()
+        for slot in [s for s in self.slot_req if done[s]]:
+            req, admitted, round0 = self.slot_req.pop(slot)
+            self.results.append(Result(req.rid, req.arrival, admitted, now, round0, self.rounds,
+                                       self.batch.state(slot), float(t[slot])))
+            self.batch.clear_case(slot)
+            self.free.append(slot)
+            self.used -= req.nbytes
+            changed = True
+        return changed
+
+    def _admit(self, now: float) -> list[int]:
+        new = []
+        while self.queue and self.queue[0].arrival <= now and self.free:
+            req = self.queue[0]
+            if self.used + req.nbytes > self.budget:
+                break  # first-come first-served: wait for memory
+            heapq.heappop(self.queue)
+            slot = self.free.pop(0)
+            self.batch.set_case(slot, req.cfg, req.U0, t0=0.0, t_end=req.t_end)
+            self.used += req.nbytes
+            self.slot_req[slot] = (req, now, self.rounds)
+            new.append(slot)
+        self.peak_used = max(self.peak_used, self.used)
+        self.peak_resident = max(self.peak_resident, len(self.slot_req))
+        return new
+
+    def run(self, until_empty: bool = True, max_rounds: int = 10 ** 9, log=None):
+        """Serve until every submitted request has finished."""
+        t_start = self.clock()
+        while self.rounds < max_rounds:
+            now = self.clock()
+            changed = self._retire()
+            new = self._admit(now)
+            if changed or new:
+                self.batch.commit(graph_steps=self.round_steps if self.slot_req else 0)
+                if new:
+                    self.batch.prepare(new)
+            if not self.slot_req:
+                if not self.queue:
+                    break
+                # idle until the next arrival
+                wait = self.queue[0].arrival - self.clock()
+                if wait > 0:
+                    self.sleep(min(wait, 0.05))
+                continue
+            self.batch.step()
+            self.rounds += 1
+            if log and self.rounds % 200 == 0:
+                log(f"  round {self.rounds}: resident {len(self.slot_req)}, queued {len(self.queue)}, "
+                    f"done {len(self.results)}, HBM {self.used / 1e9:.2f} GB")
+        return self.clock() - t_start
+
+    def close(self):
+        self.batch.close()
+
+
+# ---------------------------------------------------------------------------
+# The serving workload
+# ---------------------------------------------------------------------------
+T_END_DROPLET = 0.05
+T_END_TRIPLE = 0.5
+
+
+def make_request(rng, rid: int, arrival: float, scheme=SCHEME_MP5, precision=PREC_MIXED):
+    """Log-uniform NX in [128, 2048] (multiple of 32), NY = clamp(NX/2, 128, 1024);
+    half shock-droplet cases (droplets ~ area, at least one), half triple points."""
+    nx = int(round(math.exp(rng.uniform(math.log(128), math.log(2048))) / 32.0)) * 32
+    nx = min(max(nx, 128), 2048)
+    ny = min(max(int(round(nx / 2 / 32.0)) * 32, 128), 1024)
+    if rng.uniform() < 0.5:
+        from .config import Grid
+
+        grid = Grid(nx, ny, 0.0, 4.0, 0.0, 2.0)
+        n = max(1, int(round(40 * nx * ny / (4096 * 2048))))
+        drops = cases.seeded_droplets(rng, n, 0.05, 0.15, 0.9, 3.9, 0.05, 1.95, 4.0 * grid.dx,
+                                      n_cavity=n // 4)
+        U0 = cases.shock_droplets(grid, cases.WATER_ND, 2.4, 0.8, drops)
+        cfg = CaseConfig(grid, cases.WATER_ND, scheme, precision, name=f"droplets_{nx}x{ny}")
+        t_end = T_END_DROPLET
+    else:
+        cfg, U0 = cases.triple_point(nx, ny, scheme=scheme)
+        cfg = CaseConfig(cfg.grid, cfg.eos, scheme, precision, name=cfg.name)
+        t_end = T_END_TRIPLE
+    return Request(arrival, rid, cfg, U0, t_end)
+
+
+def poisson_workload(n: int, rate: float, seed: int = 5, t0: float = 0.0, **kw):
+    rng = np.random.default_rng(seed)
+    t = t0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        out.append(make_request(rng, k, t, **kw))
+    return out
+
+
+def with_arrivals(reqs, rate: float | None, base: float = 0.0, precision: int | None = None):
+    """Copies of `reqs` whose arrivals are rescaled to a Poisson rate (`rate`
+    requests/s; the workload was drawn at rate 1) or all at `base` (rate None)."""
+    out = []
+    for r in reqs:
+        cfg = r.cfg if precision is None else dataclasses.replace(r.cfg, precision=precision)
+        arrival = base if rate is None else base + r.arrival / rate
+        out.append(Request(arrival, r.rid, cfg, r.U0, r.t_end))
+    return out
+
+
+def _stats(results, elapsed: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {
+        "completed": int(len(lat)),
+        "elapsed_s": float(elapsed),
+        "cases_per_hour": float(len(lat) / elapsed * 3600.0),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+    }
+
+
+def serve_plain(reqs, clock=time.perf_counter, keep=None, log=None):
+    """The plain solver serving a queue: one request at a time in arrival order,
+    each with its own Solver (set_state, prepare, the reference's time loop to
+    t_end, result copied to the host).  `reqs` arrivals are offsets from the
+    start.  `keep(rid, state)` receives each result."""
+    import torch
+
+    from .solver import Solver
+
+    order = sorted(reqs)
+    t0 = clock()
+    results = []
+    for r in order:
+        now = clock() - t0
+        if now < r.arrival:
+            time.sleep(r.arrival - now)
+        start = clock() - t0
+        s = Solver(r.cfg)
+        s.set_state(r.U0)
+        s.prepare()
+        s.advance_to(r.t_end, log=None)
+        U = s.state()
+        torch.cuda.synchronize()
+        s.close()
+        done = clock() - t0
+        if keep:
+            keep(r.rid, U)
+        results.append(Result(r.rid, r.arrival, start, done, 0, 0, None, s.time))
+        if log and len(results) % 25 == 0:
+            log(f"plain: {len(results)} of {len(order)} served, {done:.1f} s")
+    return results, _stats(results, clock() - t0)
+
+
+def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int = 64,
+                  hbm_budget: float = 64e9, round_steps: int = 10, tuning: Tuning | None = None,
+                  log=None):
+    """Run `reqs` (arrivals as offsets from the start) through a Server."""
+    srv = Server(scheme=scheme, precision=precision, max_slots=max_slots, hbm_budget=hbm_budget,
+                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 16))
+    base = srv.clock()
+    for r in reqs:
+        srv.submit(Request(base + r.arrival, r.rid, r.cfg, r.U0, r.t_end))
+    elapsed = srv.run(log=log)
+    for r in srv.results:
+        r.arrival -= base
+        r.admitted -= base
+        r.finished -= base
+    st = _stats(srv.results, elapsed)
+    st.update(rounds=srv.rounds, peak_resident=srv.peak_resident,
+              peak_hbm_gb=round(srv.peak_used / 1e9, 3))
+    results = srv.results
+    srv.close()
+    return results, st
