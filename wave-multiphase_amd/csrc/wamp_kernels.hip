@@ -581,20 +581,38 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
 // pipeline.  One wave owns 64 consecutive j (58 output columns plus a 3-lane
 // halo each side: the update of a cell needs theta2 at +-1, theta1 at +-2,
 // theta0 at +-3 cells) and marches along i.  At front row f it forms
-// theta0(f), theta1(f-1), theta2(f-2) and the update of row f-3.  Every
-// face value is fetched from HBM once per use-group and kept in registers
-// while it is live:
-//   x-faces  components {0,1} of row f, {5,6} of row f-1 and all of row f-2
-//            are loaded at front f; rows f-4..f-2 (all components) stay in
-//            registers for theta2 and the update;
-//   y-faces  the face above the cell is loaded; the face below is the
-//            neighbouring lane's (lane shuffle), except for theta0, whose
-//            lower faces are loaded so that theta0 is exact on every lane;
-//            row f-2's values stay in registers for the update one front later.
+// theta0(f), theta1(f-1), theta2(f-2) and the update of row f-3.
+//
+// Every face and state value is read from memory once per wave: a value
+// needed again at a later front is carried, in registers or in a wave-private
+// LDS ring indexed by row, instead of being re-fetched (a re-fetch one to
+// three fronts later misses the L2 once all resident waves' windows are
+// counted):
+//   x-faces  row f: components {0,1} (theta0); row f-1: {5,6} (theta1); row
+//            f-2: {2,3,4} (theta2).  Rows f-2..f of {0,1} and f-2..f-1 of
+//            {5,6} stay in registers; rows f-3..f-2 of all components too.
+//   y-faces  row f: {0,1} (theta0); row f-1: {5,6} (theta1); row f-2:
+//            {2,3,4} (theta2).  {0,1} and {5,6} wait in LDS until theta2.
+//            The face below a cell is the neighbouring lane's (shuffle),
+//            except for theta0, whose lower faces are loaded so theta0 is
+//            exact on every lane.
+//   state    U0, U1 at theta0, U5 at theta1, U2..U4 at theta2; all wait in
+//            LDS until the update one front after theta2.
+// Values of rows the wave has not visited yet read as zero (the ring is
+// cleared first); they only feed thetas of rows that no output depends on.
 // ========================================================================
 constexpr int MW = 64;
 constexpr int MH = 3;
 constexpr int MJ = MW - 2 * MH;
+
+// LDS carry ring, doubles per lane: [slot][value] blocks of 64 lanes.
+constexpr int LC_Y01 = 0;    // y-faces H0 H1 L0 L1, 3 row slots
+constexpr int LC_Y56 = 12;   // y-faces H5 H6 L5 L6, 2 row slots
+constexpr int LC_U01 = 20;   // U0 U1, 4 row slots
+constexpr int LC_U5 = 28;    // U5, 3 row slots
+constexpr int LC_U234 = 31;  // U2 U3 U4, 2 row slots
+constexpr int LU_CARRY = 37;
+constexpr int LU_WAVES = 4;  // waves per limiter block
 
 __device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, MW); }
 __device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, MW); }
@@ -607,13 +625,13 @@ __device__ __forceinline__ double blend3(double H, double L, const double* th, i
 }
 
 // One wave's work item: columns strip*58-3 .. +64, rows seg*SEG .. +SEG.
-// Returns this lane's CFL minimum of the updated cells (1e30 if none or if
-// want_dt is false).
+// `cw` is this wave's LU_CARRY x 64 doubles of LDS.  Returns this lane's CFL
+// minimum of the updated cells (1e30 if none or if want_dt is false).
 template <int SEG>
 __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const State& Un,
                                           const State& U, const Faces& F, const State& Uo,
                                           double alpha, double beta, double dt, int repair,
-                                          bool want_dt, int strip, int seg) {
+                                          bool want_dt, int strip, int seg, double* cw) {
   const int lane = threadIdx.x & (MW - 1);
   const int j = G.g + strip * MJ - MH + lane;
   const int i0 = G.g + seg * SEG;
@@ -628,9 +646,13 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   auto row_in = [&](int r) { return r >= G.g && r < G.g + G.nx; };
   auto xrow_in = [&](int r) { return r >= G.g - 1 && r < G.g + G.nx; };  // x-face r+1/2 exists
   auto at = [&](int r) { return (size_t)r * ld + (size_t)j; };
+  auto lds = [&](int q) -> double& { return cw[q * MW + lane]; };
   // face component m: high F[base + m], low F[base + 14 + m]; base 0 = x, 7 = y
 
-  double x01[2][4] = {};   // x-face rows f-1, f: H0 H1 L0 L1
+#pragma unroll
+  for (int q = 0; q < LU_CARRY; ++q) lds(q) = 0.0;
+
+  double x01[3][4] = {};   // x-face rows f-2, f-1, f: H0 H1 L0 L1
   double x56[2][4] = {};   // x-face rows f-2, f-1: H5 H6 L5 L6
   double xa[2][14] = {};   // x-face rows f-3, f-2: H0..6 L0..6
   double F2xm[7] = {};     // final flux at x-face row f-4 (below the update row)
@@ -641,16 +663,22 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   double local_dt = 1.0e30;
 
   for (int f = i0 - 3; f < i_end + 3; ++f) {
+    const int s3f = f % 3, s3m1 = (f + 2) % 3, s3m2 = (f + 1) % 3;  // slots of rows f, f-1, f-2
+    const int s2m1 = (f + 1) & 1, s2m2 = f & 1;                      // rows f-1, f-2 (f-3 = f-1)
+    const int s4f = f & 3, s4m2 = (f + 2) & 3, s4m3 = (f + 1) & 3;  // rows f, f-2, f-3
     // ---------------- x-face loads of this front ----------------
     {
       const bool ok = xrow_in(f) && col_in;
       const size_t c = at(f);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) x01[0][q] = x01[1][q];
-      x01[1][0] = ok ? F[0][c] : 0.0;
-      x01[1][1] = ok ? F[1][c] : 0.0;
-      x01[1][2] = ok ? F[14][c] : 0.0;
-      x01[1][3] = ok ? F[15][c] : 0.0;
+      for (int q = 0; q < 4; ++q) {
+        x01[0][q] = x01[1][q];
+        x01[1][q] = x01[2][q];
+      }
+      x01[2][0] = ok ? F[0][c] : 0.0;
+      x01[2][1] = ok ? F[1][c] : 0.0;
+      x01[2][2] = ok ? F[14][c] : 0.0;
+      x01[2][3] = ok ? F[15][c] : 0.0;
     }
     {
       const bool ok = xrow_in(f - 1) && col_in;
@@ -663,12 +691,21 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       x56[1][3] = ok ? F[20][c] : 0.0;
     }
     {
+      // row f-2: components 0, 1 and 5, 6 are already held (same predicate)
       const bool ok = xrow_in(f - 2) && col_in;
       const size_t c = at(f - 2);
 #pragma unroll
       for (int m = 0; m < 14; ++m) xa[0][m] = xa[1][m];
+      xa[1][0] = x01[0][0];
+      xa[1][1] = x01[0][1];
+      xa[1][7] = x01[0][2];
+      xa[1][8] = x01[0][3];
+      xa[1][5] = x56[0][0];
+      xa[1][6] = x56[0][1];
+      xa[1][12] = x56[0][2];
+      xa[1][13] = x56[0][3];
 #pragma unroll
-      for (int m = 0; m < 7; ++m) {
+      for (int m = 2; m < 5; ++m) {
         xa[1][m] = ok ? F[m][c] : 0.0;
         xa[1][7 + m] = ok ? F[14 + m][c] : 0.0;
       }
@@ -679,21 +716,33 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
     {
       const size_t c = at(f);
       const bool ok = row_in(f) && col_in;
+      const bool yok = row_in(f) && yface_in;
+      double yc[4];
+      yc[0] = yok ? F[7][c] : 0.0;
+      yc[1] = yok ? F[8][c] : 0.0;
+      yc[2] = yok ? F[21][c] : 0.0;
+      yc[3] = yok ? F[22][c] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) lds(LC_Y01 + 4 * s3f + q) = yc[q];
+      double u01[2] = {0.0, 0.0};
       if (ok) {
         double l[6], h[6];
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
           // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
-          double yuh = F[7 + m][c], ydh = F[7 + m][c - 1];
-          double yul = F[21 + m][c], ydl = F[21 + m][c - 1];
-          double rh = (x01[1][m] - x01[0][m]) * inv_dx + (yuh - ydh) * inv_dy;
-          double rl = (x01[1][2 + m] - x01[0][2 + m]) * inv_dx + (yul - ydl) * inv_dy;
+          double yuh = yc[m], ydh = F[7 + m][c - 1];
+          double yul = yc[2 + m], ydl = F[21 + m][c - 1];
+          double rh = (x01[2][m] - x01[1][m]) * inv_dx + (yuh - ydh) * inv_dy;
+          double rl = (x01[2][2 + m] - x01[1][2 + m]) * inv_dx + (yul - ydl) * inv_dy;
           double u = U.f[m][c];
+          u01[m] = u;
           l[m] = u - dt * rl;
           h[m] = u - dt * rh;
         }
         th = theta_stage<0>(e, l, h);
       }
+      lds(LC_U01 + 2 * s4f) = u01[0];
+      lds(LC_U01 + 2 * s4f + 1) = u01[1];
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) t0[k] = t0[k + 1];
@@ -715,12 +764,15 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       yu[1] = yok ? F[13][c] : 0.0;
       yu[2] = yok ? F[26][c] : 0.0;
       yu[3] = yok ? F[27][c] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) lds(LC_Y56 + 4 * s2m1 + q) = yu[q];
       double yd[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) yd[q] = lane_up(yu[q]);
       double th1 = 1.0;
+      double al = 0.0;
       if (ok) {
-        const double al = U.f[5][c];
+        al = U.f[5][c];
         double Hxp5 = blend3(x56[1][0], x56[1][2], &tf[1], 1);
         double Hxm5 = blend3(x56[0][0], x56[0][2], &tf[0], 1);
         double Hyp5 = blend3(yu[0], yu[2], &tf[3], 1);
@@ -738,6 +790,7 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
         h[5] = al - dt * rh;
         th1 = theta_stage<1>(e, l, h);
       }
+      lds(LC_U5 + s3m1) = al;
 #pragma unroll
       for (int k = 0; k < 3; ++k) t1[k] = t1[k + 1];
       t1[3] = th1;
@@ -750,8 +803,16 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const size_t c = at(r);
       const bool ok = row_in(r) && col_in;
       const bool yok = row_in(r) && yface_in;
+      yn[0] = lds(LC_Y01 + 4 * s3m2 + 0);
+      yn[1] = lds(LC_Y01 + 4 * s3m2 + 1);
+      yn[7] = lds(LC_Y01 + 4 * s3m2 + 2);
+      yn[8] = lds(LC_Y01 + 4 * s3m2 + 3);
+      yn[5] = lds(LC_Y56 + 4 * s2m2 + 0);
+      yn[6] = lds(LC_Y56 + 4 * s2m2 + 1);
+      yn[12] = lds(LC_Y56 + 4 * s2m2 + 2);
+      yn[13] = lds(LC_Y56 + 4 * s2m2 + 3);
 #pragma unroll
-      for (int m = 0; m < 7; ++m) {
+      for (int m = 2; m < 5; ++m) {
         yn[m] = yok ? F[7 + m][c] : 0.0;
         yn[7 + m] = yok ? F[21 + m][c] : 0.0;
       }
@@ -768,8 +829,19 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 #pragma unroll
       for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
       double th2 = 1.0;
+      double u234[3] = {0.0, 0.0, 0.0};
       if (ok) {
-        const double al = U.f[5][c];
+        double uc[6];
+        uc[0] = lds(LC_U01 + 2 * s4m2);
+        uc[1] = lds(LC_U01 + 2 * s4m2 + 1);
+        uc[2] = U.f[2][c];
+        uc[3] = U.f[3][c];
+        uc[4] = U.f[4][c];
+        uc[5] = lds(LC_U5 + s3m2);
+        u234[0] = uc[2];
+        u234[1] = uc[3];
+        u234[2] = uc[4];
+        const double al = uc[5];
         // component 6 (contact velocity) first: it enters component 5's residual
         const double s_hm = blend3(xa[0][6], xa[0][13], tf[0], 2);
         const double s_hp = blend3(xa[1][6], xa[1][13], tf[1], 2);
@@ -788,12 +860,14 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
             rh = rh - al * (s_hp - s_hm) * inv_dx - al * (s_hu - s_hd) * inv_dy;
             rl = rl - al * (xa[1][13] - xa[0][13]) * inv_dx - al * (yn[13] - ydn[13]) * inv_dy;
           }
-          const double u = U.f[m][c];
+          const double u = uc[m];
           l[m] = u - dt * rl;
           h[m] = u - dt * rh;
         }
         th2 = theta_stage<2>(e, l, h);
       }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) lds(LC_U234 + 3 * s2m2 + q) = u234[q];
 #pragma unroll
       for (int k = 0; k < 2; ++k) t2[k] = t2[k + 1];
       t2[2] = th2;
@@ -825,14 +899,22 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 #pragma unroll
       for (int m = 0; m < 7; ++m) yd[m] = lane_up(F2yu[m]);
       if (r >= i0 && r < i_end && out_lane) {
-        const double ap = U.f[5][c];
+        // row f-3 is the previous front's theta2 row (s2m1 is its U2..U4 slot)
+        double uc[6];
+        uc[0] = lds(LC_U01 + 2 * s4m3);
+        uc[1] = lds(LC_U01 + 2 * s4m3 + 1);
+        uc[2] = lds(LC_U234 + 3 * s2m1);
+        uc[3] = lds(LC_U234 + 3 * s2m1 + 1);
+        uc[4] = lds(LC_U234 + 3 * s2m1 + 2);
+        uc[5] = lds(LC_U5 + s3f);  // (f-3) % 3 == f % 3
+        const double ap = uc[5];
         double un[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
           double res = (F2xp[m] - F2xm[m]) * inv_dx + (F2yu[m] - yd[m]) * inv_dy;
           if (m == 5)
             res = res - ap * (F2xp[6] - F2xm[6]) * inv_dx - ap * (F2yu[6] - yd[6]) * inv_dy;
-          un[m] = alpha * Un.f[m][c] + beta * (U.f[m][c] - dt * res);
+          un[m] = alpha * Un.f[m][c] + beta * (uc[m] - dt * res);
         }
         if (repair) repair_cell(e, un);
         un[5] = clamp_al(un[5]);
@@ -861,11 +943,12 @@ template <int SEG>
 __global__ void __launch_bounds__(256, 2)
 k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
            const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+  __shared__ double carry[LU_WAVES * LU_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
   const int seg = (int)blockIdx.y * (blockDim.x / MW) + wave;
   if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
   double v = lu_wave<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
-                          (int)blockIdx.x, seg);
+                          (int)blockIdx.x, seg, carry + wave * (LU_CARRY * MW));
   if (dt_acc) wave_min_atomic(v, dt_acc);
 }
 
@@ -1031,6 +1114,7 @@ k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, in
            BatchScal sc, int stage, int repair) {
   // the wave index is uniform across the wave: keep it (and everything derived
   // from it: case, descriptor, rows) in scalar registers
+  __shared__ double carry[LU_WAVES * LU_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);
   const int it = blockIdx.x * (blockDim.x / MW) + wave;
   if (it >= item_off[ncase]) return;
@@ -1048,7 +1132,7 @@ k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, in
     Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
   }
   double v = lu_wave<SEG>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair, stage == 2,
-                          li % c.strips, li / c.strips);
+                          li % c.strips, li / c.strips, carry + wave * (LU_CARRY * MW));
   if (stage == 2) wave_min_atomic(v, sc.acc + k);
 }
 
