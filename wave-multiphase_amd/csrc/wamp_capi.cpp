@@ -17,14 +17,11 @@
 
 #include "wamp.h"
 
-namespace wamp {
-struct Eos {
-  double g1, p1, g2, p2;
-};
-}  // namespace wamp
 #include "wamp_types.h"
 
 using namespace wamp;
+
+static Eos make_eos(const wamp_eos& e) { return make_eos_values(e.gamma1, e.pinf1, e.gamma2, e.pinf2); }
 
 extern "C" {
 hipError_t wamp_launch_faces(const Geo*, const Eos*, const State*, const Faces*, int base,
@@ -149,7 +146,7 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   c->geo.dx = grid->dx;
   c->geo.dy = grid->dy;
   for (int k = 0; k < 4; ++k) c->geo.bc[k] = grid->bc[k];
-  c->eos = Eos{eos->gamma1, eos->pinf1, eos->gamma2, eos->pinf2};
+  c->eos = make_eos(*eos);
   c->scheme = *scheme;
   c->precision = precision;
   c->t1 = std::tanh(1.8 * 0.5);
@@ -534,7 +531,7 @@ int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, cons
   d = CaseDesc();
   d.geo = Geo{grid->nx, grid->ny, grid->g, grid->ld, grid->dx, grid->dy,
               {grid->bc[0], grid->bc[1], grid->bc[2], grid->bc[3]}};
-  d.eos = Eos{eos->gamma1, eos->pinf1, eos->gamma2, eos->pinf2};
+  d.eos = make_eos(*eos);
   for (int m = 0; m < 6; ++m) d.u[0][m] = U[m];
   d.f = Faces{p, (long long)plane};
   p += 28 * plane;

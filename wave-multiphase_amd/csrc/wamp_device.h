@@ -19,11 +19,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wamp_types.h"
+
 namespace wamp {
 
-struct Eos {
-  double g1, p1, g2, p2;
-};
+
 
 // Warp builtin semantics: comparison selects, sign(0) = +1.
 template <typename T> __device__ __forceinline__ T wmin(T a, T b) { return a < b ? a : b; }
@@ -46,6 +46,19 @@ __device__ __forceinline__ float th(float x) {
   float t = __expf(-2.0f * fabsf(x));
   float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
   return x < 0.0f ? -r : r;
+}
+
+// a / b from y = RN(1/b) with one correction step (Markstein 1990; Muller et
+// al., Handbook of Floating-Point Arithmetic, the FMA-based division
+// theorem): q = RN(a y), r = a - b q (exact with an FMA), RN(q + r y) is the
+// correctly rounded quotient for finite a, b with no overflow or underflow,
+// i.e. bitwise a / b.  Three instructions instead of the IEEE divide
+// sequence when y is shared (a common denominator or a host constant).
+// Callers use it only where b is finite and non-zero.
+__device__ __forceinline__ double quotient(double a, double y, double b) {
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
 }
 
 // Reference constants, SC:44-70.
@@ -73,6 +86,15 @@ __device__ __forceinline__ void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
   T a2 = T(1) - a1;
   gb = dv(a1, T(e.g1) - T(1)) + dv(a2, T(e.g2) - T(1));
   pb = dv(a1 * T(e.g1) * T(e.p1), T(e.g1) - T(1)) + dv(a2 * T(e.g2) * T(e.p2), T(e.g2) - T(1));
+}
+// FP64: the four divisions by the constants g-1 use the host reciprocals
+// (bitwise the same quotients; gamma > 1 is checked at context creation).
+template <>
+__device__ __forceinline__ void mix_gb_pb<double>(const Eos& e, double a1, double& gb, double& pb) {
+  const double a2 = 1.0 - a1;
+  const double d1 = e.g1 - 1.0, d2 = e.g2 - 1.0;
+  gb = quotient(a1, e.r1, d1) + quotient(a2, e.r2, d2);
+  pb = quotient(a1 * e.g1 * e.p1, e.r1, d1) + quotient(a2 * e.g2 * e.p2, e.r2, d2);
 }
 // mix_gamma_pinf SC:90-95
 template <typename T>
@@ -141,9 +163,10 @@ __device__ __forceinline__ double rho_ct2_cons(const Eos& e, double m1, double m
   double a1 = clamp01_pp(a1_in);
   if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
   if (m2 < PP_RHO || isnan_(m2)) m2 = PP_RHO;
-  double rho = m1 + m2;
-  double uu = mx / rho;
-  double vv = my / rho;
+  double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
+  const double y = 1.0 / rho;
+  double uu = quotient(mx, y, rho);
+  double vv = quotient(my, y, rho);
   double kin = 0.5 * rho * (uu * uu + vv * vv);
   double g, pi;
   mix_g_pi(e, a1, g, pi);

@@ -425,9 +425,10 @@ __device__ __forceinline__ void repair_cell(const Eos& e, double* u) {
   if (isnan_(m2) || m2 < PP_RHO) m2 = PP_RHO;
   if (isnan_(mx)) mx = 0.0;
   if (isnan_(my)) my = 0.0;
-  double rho = m1 + m2;
-  double uu = mx / rho;
-  double vv = my / rho;
+  double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
+  const double y = 1.0 / rho;
+  double uu = quotient(mx, y, rho);
+  double vv = quotient(my, y, rho);
   double kin = 0.5 * rho * (uu * uu + vv * vv);
   double gb, pb;
   mix_gb_pb(e, a1, gb, pb);

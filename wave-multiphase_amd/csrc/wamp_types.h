@@ -3,6 +3,17 @@
 
 namespace wamp {
 
+// Two stiffened-gas phases.  r1, r2 are the correctly rounded reciprocals
+// 1/(g1-1), 1/(g2-1), formed on the host (make_eos); the FP64 device EOS
+// divides by g-1 through them.
+struct Eos {
+  double g1, p1, g2, p2;
+  double r1, r2;
+};
+inline Eos make_eos_values(double g1, double p1, double g2, double p2) {  // host
+  return Eos{g1, p1, g2, p2, 1.0 / (g1 - 1.0), 1.0 / (g2 - 1.0)};
+}
+
 struct Geo {  // one case's grid: interior nx x ny, g ghost layers, row stride ld
   int nx, ny, g, ld;
   double dx, dy;
