@@ -64,8 +64,8 @@ struct wamp_ctx {
   int precision;
   int device;
   double t1;          // tanh(beta_thinc / 2), SC:334
-  int lu_kind;        // 0: LDS-tile limiter/update kernel, 1: register-marching kernel
-  int lu_seg;         // rows per wave of the marching kernel
+  int lu_kind;        // 0: LDS-tile limiter/update kernel, 1: marching kernel, -1: by grid size
+  int lu_seg;         // rows per wave of the marching kernel (0: by grid size)
   double* block;      // one allocation: faces | stage buffers | dt accumulator
   int64_t bytes;
   Faces faces;
@@ -150,8 +150,8 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   c->scheme = *scheme;
   c->precision = precision;
   c->t1 = std::tanh(1.8 * 0.5);
-  c->lu_kind = 1;
-  c->lu_seg = 16;
+  c->lu_kind = -1;
+  c->lu_seg = 0;
   if (const char* v = std::getenv("WAMP_LU_KIND")) c->lu_kind = std::atoi(v);
   if (const char* v = std::getenv("WAMP_LU_SEG")) c->lu_seg = std::atoi(v);
   int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
@@ -254,12 +254,33 @@ static int faces_of(const wamp_ctx* c, double* const F[28], Faces& f) {
   return 0;
 }
 
+// Rows per wave of the marching limiter.  Each wave's 3-row warm-up and
+// drain make a segment's latency grow with its length; 16 rows is fastest
+// when the grid yields enough waves to occupy every SIMD twice (measured at
+// 4096x2048), shorter segments halve the serial march of small grids.
+static int lu_rows(const wamp_ctx* c) {
+  if (c->lu_seg) return c->lu_seg;
+  const long long strips = (c->geo.ny + 57) / 58;
+  const long long waves16 = strips * ((c->geo.nx + 15) / 16);
+  return waves16 < 2048 ? 8 : 16;  // (4 rows: 0.18 ms/step at 128x128, 8: 0.25, 16: 0.39)
+}
+
+// Limiter kernel: the LDS-tile kernel (every tile in parallel, four short
+// phases) has the lower latency on small grids, whose march would leave most
+// SIMDs idle; the marching kernel moves fewer bytes and wins from about
+// 2^18 cells (measured: 512x256 0.12 vs 0.19 ms per step, 1024x512 0.37 vs
+// 0.33, 2048x1024 1.45 vs 1.14).  Both give bitwise the same result.
+static int lu_kind(const wamp_ctx* c) {
+  if (c->lu_kind >= 0) return c->lu_kind;
+  return (long long)c->geo.nx * c->geo.ny <= (1LL << 18) ? 0 : 1;
+}
+
 static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& f,
                      const State& uo, double alpha, double beta, const double* dt_dev,
                      double* dt_acc, hipStream_t st) {
-  hipError_t e = c->lu_kind == 1
+  hipError_t e = lu_kind(c) == 1
                      ? wamp_launch_lu_march(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev,
-                                            dt_acc, c->scheme.repair, c->lu_seg, st)
+                                            dt_acc, c->scheme.repair, lu_rows(c), st)
                      : wamp_launch_limit_update(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta,
                                                 dt_dev, dt_acc, c->scheme.repair, st);
   return hip_check(e, "limit_update");

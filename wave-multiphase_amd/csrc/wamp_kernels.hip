@@ -1231,7 +1231,7 @@ hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, con
                        *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);                    \
     return hipGetLastError();                                                                \
   }
-  WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(32) WAMP_SEG(64)
+  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(32) WAMP_SEG(64)
 #undef WAMP_SEG
   return hipErrorInvalidValue;
 }
