@@ -245,6 +245,25 @@ def serving_leg(n: int, seed: int = 5, load: float = 0.7):
     }
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), or None when
+    there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            line = line.strip()
+            if line and not line.startswith("#") and "=" in line:
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("workload") != workload or vals.get("kernel") != kernel:
+        return None
+    return int(vals["traffic_bytes"])
+
+
 def _progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -361,7 +380,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces"),
             "launch_ms": round(faces_ms, 4),
             "launches": int(nl[0]),
         },
