@@ -1,0 +1,71 @@
+"""The headline case at its full size, 4096x2048 (Mach-10 water cylinder, M10):
+one SSP-RK3 step of the HIP path against the CPU oracle, then size-independent
+properties over many steps -- mirror symmetry about the cylinder's axis (the
+case and the slip walls are symmetric in y), admissibility and finiteness."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case():
+    from wamp import cases
+    from wamp.config import SCHEME_M10
+
+    return cases.mach10_cylinder(4096, 2048, scheme=SCHEME_M10)
+
+
+def test_full_size_step_matches_oracle(gpu_ok, case):
+    from oracle.oracle import Oracle
+    from wamp.solver import Solver
+
+    cfg, U0 = case
+    threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    orc = Oracle(cfg, threads=threads)
+    orc.set_state(U0)
+    orc.prepare()
+    U = orc.state()
+    dt = orc.get_dt()
+    orc.step(dt)
+    ref = orc.state()
+    del orc
+    s = Solver(cfg)
+    s.set_state(U)
+    s.set_dt(dt)
+    s.step_device()
+    got = s.state()
+    s.close()
+    si, sj = cfg.grid.interior()
+    errs = [rel_l1(got[m][si, sj], ref[m][si, sj]) for m in range(6)]
+    print("4096x2048 one step:", ["%.2e" % e for e in errs])
+    assert max(errs) <= 1e-12
+
+
+def test_full_size_symmetry_and_admissibility(gpu_ok, case):
+    from wamp.solver import Solver
+
+    cfg, U0 = case
+    s = Solver(cfg)
+    s.set_state(U0)
+    s.prepare()
+    dts = s.run(100)
+    S = s.state()
+    s.close()
+    assert np.all(dts > 0) and np.all(np.isfinite(dts))
+    si, sj = cfg.grid.interior()
+    worst = 0.0
+    for m in range(6):
+        a = S[m][si, sj]
+        assert np.isfinite(a).all(), m
+        b = a[:, ::-1] * (-1.0 if m == 3 else 1.0)  # v changes sign under the mirror
+        worst = max(worst, rel_l1(a, b) if m != 3 else float(np.abs(a - b).sum() / np.abs(S[2][si, sj]).sum()))
+    print(f"4096x2048 after 100 steps: mirror asymmetry {worst:.2e}")
+    assert worst <= 1e-10
+    ar1, ar2, al = S[0][si, sj], S[1][si, sj], S[5][si, sj]
+    assert ar1.min() > 0.0 and ar2.min() > 0.0
+    assert al.min() >= 1.0e-8 and al.max() <= 1.0 - 1.0e-8

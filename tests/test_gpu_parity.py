@@ -106,7 +106,7 @@ def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, monkeypatch):
     U = orc.state()
     dt = orc.get_dt()
     outs = {}
-    for kind, seg in (("0", "32"), ("1", "8"), ("1", "32"), ("1", "64")):
+    for kind, seg in (("0", "32"), ("1", "4"), ("1", "8"), ("1", "16"), ("1", "32"), ("1", "64")):
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)
         s = Solver(cfg)
