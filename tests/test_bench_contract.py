@@ -1,0 +1,29 @@
+"""bench.py pieces that run without a GPU: the one-GPU guard and the lookup of
+the committed rocprof traffic figure for the roofline."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+
+
+def test_traffic_record_lookup(tmp_path, monkeypatch):
+    sys.path.insert(0, ROOT)
+    import bench
+
+    v = bench.measured_traffic("mach10_cylinder_4096x2048", "k_faces")
+    assert isinstance(v, int) and v > 1_074_000_000  # above the algorithmic 128 B/cell
+    assert bench.measured_traffic("mach10_cylinder_512x256", "k_faces") is None
+    # a record for another workload is not used
+    prof = tmp_path / "profiles"
+    prof.mkdir()
+    (prof / "k_faces_traffic_x_1x1.txt").write_text("workload=other\nkernel=k_faces\ntraffic_bytes=5\n")
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    assert bench.measured_traffic("x_1x1", "k_faces") is None

@@ -64,8 +64,9 @@ class Result:
 class Server:
     def __init__(self, scheme=SCHEME_MP5, precision: int = PREC_MIXED, max_slots: int = 64,
                  hbm_budget: float = 64e9, round_steps: int = 10, tuning: Tuning | None = None,
-                 clock=time.perf_counter, sleep=time.sleep):
-        self.batch = Batch(scheme, precision, max_slots)
+                 clock=time.perf_counter, sleep=time.sleep, batch=None):
+        # `batch`: a Batch-like object to drive (the host logic's tests pass a stand-in)
+        self.batch = batch if batch is not None else Batch(scheme, precision, max_slots)
         self.scheme, self.precision = scheme, precision
         self.max_slots = max_slots
         self.budget = float(hbm_budget)
