@@ -30,6 +30,9 @@ sys.path.insert(0, os.path.join(ROOT, "wave-multiphase_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector
+VALU_SIMDS = 1024          # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9
 ALG_BYTES_PER_CELL_UPDATE = 384.0  # FP64 state: 96 + 144 + 144 B over the 3 stages
 
 
@@ -245,23 +248,52 @@ def serving_leg(n: int, seed: int = 5, load: float = 0.7):
     }
 
 
+def _record(name: str) -> dict:
+    """key=value lines of a committed measurement record under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    vals = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            for line in fh:
+                line = line.strip()
+                if line and not line.startswith("#") and "=" in line:
+                    k, v = line.split("=", 1)
+                    vals[k.strip()] = v.strip()
+    return vals
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (profiles/), or None when
     there is none for this workload."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_{workload}.txt")
-    if not os.path.exists(path):
-        return None
-    vals = {}
-    with open(path) as fh:
-        for line in fh:
-            line = line.strip()
-            if line and not line.startswith("#") and "=" in line:
-                k, v = line.split("=", 1)
-                vals[k.strip()] = v.strip()
+    vals = _record(f"{kernel}_traffic_{workload}.txt")
     if vals.get("workload") != workload or vals.get("kernel") != kernel:
         return None
     return int(vals["traffic_bytes"])
+
+
+def measured_step_work(workload: str):
+    """(FP64 FLOP, VALU wave instructions) of one FP64 step of `workload` from
+    the committed rocprofv3 instruction counts, or None."""
+    vals = _record(f"step_fp64_work_{workload}.txt")
+    if vals.get("workload") != workload:
+        return None
+    return int(vals["fp64_flop_per_step"]), int(vals["valu_wave_instr_per_step"])
+
+
+def valu_fracs(workload: str, ms_per_step: float):
+    """FP64 FLOP rate and VALU issue rate of the measured step against the
+    chip's peaks (78.6 TFLOP/s FP64; 1024 SIMDs x 2.4 GHz / 4 cycles per wave
+    instruction), from the committed per-step instruction counts."""
+    work = measured_step_work(workload)
+    if work is None:
+        return None
+    flop, valu = work
+    t = ms_per_step / 1e3
+    return {"fp64_tflops": round(flop / t / 1e12, 3),
+            "fp64_frac": round(flop / t / (FP64_PEAK_TFLOPS * 1e12), 4),
+            "valu_issue_frac": round(valu * 4 / (VALU_SIMDS * CLOCK_HZ * t), 4),
+            "source": f"profiles/step_fp64_work_{workload}.txt"}
 
 
 def _progress(msg: str):
@@ -386,6 +418,7 @@ def main(argv=None):
         },
         "kernels_ms": {"k_faces": round(faces_ms, 4), "k_limit_update": round(lu_ms, 4)},
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),
+        "step_valu_frac": valu_fracs(f"mach10_cylinder_{g.nx}x{g.ny}", t_ms / K),
         "state_finite": finite,
         "dt_last": dt_now,
     }
