@@ -122,6 +122,42 @@ int wamp_get_dt(wamp_ctx *ctx, double *const U[6], double *dt_dev, void *stream)
 int wamp_step(wamp_ctx *ctx, double *const U[6], const double *dt_dev, double *dt_next_dev,
               void *stream);
 
+/* ---- Initial conditions on the device -------------------------------------
+   Replace the reference's initial-condition kernel (ic_kernel2, SC:779-837 /
+   M10:736-790), a dimensional sharp-interface setup, with the nondimensional
+   cases the solver is measured on, built in device memory (the host-side
+   numpy versions are wamp/cases.py).  Both fill U's (nx+2g) x ld planes:
+   interior cells from the case, all other cells zero.  x_i = xmin +
+   (i-g+1/2) dx, y likewise (SC:1832). */
+
+/* Planar shock at shock_x moving +x into air at rest (rho_air, p0) holding
+   water droplets (rho_water): drops[4k..4k+3] = (xc, yc, r, r_cavity), a
+   device array; r_cavity > 0 carves a concentric air cavity.  Volume
+   fraction a1 = max over droplets of 1/2 (1 - tanh((d - r)/width)) (times
+   1 - 1/2 (1 - tanh((d - r_cavity)/width)) with a cavity), clipped to
+   [1e-6, 1 - 1e-6]; behind the shock (x < shock_x) the air state is
+   (rho_post, p_post, u_post). */
+typedef struct wamp_droplet_ic {
+    double xmin, ymin;
+    double shock_x, rho_post, p_post, u_post;
+    double rho_water, rho_air, p0;
+    double width;
+    const double *drops;
+    int32_t n_drops;
+} wamp_droplet_ic;
+int wamp_init_droplets(const wamp_grid *grid, const wamp_eos *eos, const wamp_droplet_ic *ic,
+                       double *const U[6], void *stream);
+
+/* Three constant states at rest: x < x_split (left), x >= x_split and
+   y >= y_split (top) in phase 1 (a1 = 1 - 1e-6), the rest (bottom) in
+   phase 2 (a1 = 1e-6) -- the gas-gas triple point. */
+typedef struct wamp_triple_ic {
+    double xmin, ymin, x_split, y_split;
+    double rho_left, p_left, rho_top, p_top, rho_bottom, p_bottom;
+} wamp_triple_ic;
+int wamp_init_triple_point(const wamp_grid *grid, const wamp_eos *eos, const wamp_triple_ic *ic,
+                           double *const U[6], void *stream);
+
 /* ---- Batches ------------------------------------------------------------
    No counterpart in the reference (one case per script run).  A batch
    advances many independent cases -- any grids and EOS, one scheme and

@@ -39,8 +39,13 @@ def _requests(precision):
         req(2, 3.0, cases.batch_case(rng, scheme=SCHEME_MP5, grids=[(80, 64)]), 0.03),
         req(3, 5.0, cases.batch_case(rng, scheme=SCHEME_MP5, grids=[(130, 40)]), 0.04),
         req(4, 6.0, cases.triple_point(64, 32, scheme=SCHEME_MP5), 0.3),
-        # arrives after everything else finished: the loop idles, then serves it
-        req(5, 1000.0, cases.shock_cylinder(64, 32, scheme=SCHEME_MP5), 0.02),
+        # arrives after everything else finished: the loop idles, then serves it;
+        # its initial state is built in device memory at admission
+        serve.Request(1000.0, 5, dataclasses.replace(cases.shock_cylinder(64, 32, scheme=SCHEME_MP5)[0],
+                                                     precision=precision), None, 0.02,
+                      build=lambda: cases.shock_droplets_device(
+                          cases.shock_cylinder(64, 32)[0].grid, cases.WATER_ND, 2.4, 0.8,
+                          [(1.5, 1.0, 0.5, 0.0)])),
     ]
 
 
@@ -48,7 +53,7 @@ def _alone(req):
     from wamp.solver import Solver
 
     s = Solver(req.cfg)
-    s.set_state(req.U0)
+    s.set_state(req.state0())
     s.prepare()
     s.advance_to(req.t_end, log=None)
     out, t = s.state(), s.time

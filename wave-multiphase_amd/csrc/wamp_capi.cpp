@@ -44,6 +44,10 @@ hipError_t wamp_launch_enforce(const Geo*, const Eos*, const State*, hipStream_t
 hipError_t wamp_launch_cons_to_prim(const Geo*, const Eos*, const State*, double*, double*,
                                     double*, hipStream_t);
 hipError_t wamp_launch_bc(const Geo*, const State*, double*, double*, double*, hipStream_t);
+hipError_t wamp_launch_ic_droplets(const Geo*, const Eos*, const State*, const DropletIC*,
+                                  hipStream_t);
+hipError_t wamp_launch_ic_triple(const Geo*, const Eos*, const State*, const TripleIC*,
+                                 hipStream_t);
 hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int ncase,
                                    const BatchScal* sc, int stage, int base, int shear,
                                    int face_pp, int vf_always, double t1, int precision, int tile,
@@ -355,6 +359,47 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
     if (int rc = mark(2 * s + 2)) return rc;
   }
   return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, c->scheme.cfl, st), "dt");
+}
+
+// ---- initial conditions on the device ------------------------------------
+static int geo_of(const wamp_grid* grid, Geo& G) {
+  if (!grid) return fail(-1, "grid is NULL");
+  if (grid->nx < 1 || grid->ny < 1 || grid->g < 4 || grid->ld < grid->ny + 2 * grid->g ||
+      !(grid->dx > 0.0) || !(grid->dy > 0.0))
+    return fail(-3, "invalid grid");
+  G = Geo{grid->nx, grid->ny, grid->g, grid->ld, grid->dx, grid->dy,
+          {grid->bc[0], grid->bc[1], grid->bc[2], grid->bc[3]}};
+  return 0;
+}
+
+int wamp_init_droplets(const wamp_grid* grid, const wamp_eos* eos, const wamp_droplet_ic* ic,
+                       double* const U[6], void* stream) {
+  Geo G;
+  if (int rc = geo_of(grid, G)) return rc;
+  if (!eos || !ic) return fail(-1, "eos and ic must be non-NULL");
+  if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  if (ic->n_drops < 0 || (ic->n_drops > 0 && !ic->drops)) return fail(-2, "drops is NULL");
+  if (!(ic->width > 0.0)) return fail(-3, "width must be > 0");
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  DropletIC d{ic->xmin, ic->ymin, ic->shock_x, ic->rho_post, ic->p_post, ic->u_post,
+              ic->rho_water, ic->rho_air, ic->p0, ic->width, ic->drops, ic->n_drops};
+  const Eos e = make_eos(*eos);
+  return hip_check(wamp_launch_ic_droplets(&G, &e, &s, &d, (hipStream_t)stream), "ic_droplets");
+}
+
+int wamp_init_triple_point(const wamp_grid* grid, const wamp_eos* eos, const wamp_triple_ic* ic,
+                           double* const U[6], void* stream) {
+  Geo G;
+  if (int rc = geo_of(grid, G)) return rc;
+  if (!eos || !ic) return fail(-1, "eos and ic must be non-NULL");
+  if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  State s;
+  if (int rc = state_of(U, s, "U")) return rc;
+  TripleIC t{ic->xmin, ic->ymin, ic->x_split, ic->y_split, ic->rho_left, ic->p_left,
+             ic->rho_top, ic->p_top, ic->rho_bottom, ic->p_bottom};
+  const Eos e = make_eos(*eos);
+  return hip_check(wamp_launch_ic_triple(&G, &e, &s, &t, (hipStream_t)stream), "ic_triple");
 }
 
 int wamp_timing_enable(wamp_ctx* c, int32_t max_steps) {

@@ -1172,6 +1172,83 @@ __global__ void k_batch_dt0(double* acc, BatchScal sc, int k, double cfl) {
 }
 
 // ========================================================================
+// initial conditions on the device
+// ========================================================================
+// Conserved state from primitives with two phase densities (the stiffened-gas
+// energy gb p + pb + kin); divisions by g-1 are IEEE divides as in the host
+// restatement of the same formula.
+__device__ __forceinline__ void ic_store(const Geo& G, const Eos& e, const State& U, size_t c,
+                                         double a1, double rho1, double rho2, double u, double v,
+                                         double p) {
+  const double ar1 = a1 * rho1;
+  const double ar2 = (1.0 - a1) * rho2;
+  const double rho = ar1 + ar2;
+  const double a2 = 1.0 - a1;
+  const double gb = a1 / (e.g1 - 1.0) + a2 / (e.g2 - 1.0);
+  const double pb = a1 * e.g1 * e.p1 / (e.g1 - 1.0) + a2 * e.g2 * e.p2 / (e.g2 - 1.0);
+  const double E = gb * p + pb + 0.5 * rho * (u * u + v * v);
+  U.f[0][c] = ar1;
+  U.f[1][c] = ar2;
+  U.f[2][c] = rho * u;
+  U.f[3][c] = rho * v;
+  U.f[4][c] = E;
+  U.f[5][c] = a1;
+}
+
+// One padded cell per thread over (nx+2g) x ld; cells outside the interior are zero.
+__global__ void __launch_bounds__(256) k_ic_droplets(Geo G, Eos e, State U, DropletIC ic) {
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long n = (long long)(G.nx + 2 * G.g) * G.ld;
+  if (k >= n) return;
+  const int i = (int)(k / G.ld), j = (int)(k % G.ld);
+  const size_t c = (size_t)k;
+  if (i < G.g || i >= G.g + G.nx || j < G.g || j >= G.g + G.ny) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) U.f[m][c] = 0.0;
+    return;
+  }
+  const double x = ic.xmin + ((double)(i - G.g) + 0.5) * G.dx;
+  const double y = ic.ymin + ((double)(j - G.g) + 0.5) * G.dy;
+  double water = 0.0;
+  for (int q = 0; q < ic.n_drops; ++q) {
+    const double xc = ic.drops[4 * q], yc = ic.drops[4 * q + 1];
+    const double r = ic.drops[4 * q + 2], rc = ic.drops[4 * q + 3];
+    const double dx = x - xc, dy = y - yc;
+    const double d = sqrt(dx * dx + dy * dy);
+    double a = 0.5 * (1.0 - tanh((d - r) / ic.width));
+    if (rc > 0.0) a = a * (1.0 - 0.5 * (1.0 - tanh((d - rc) / ic.width)));
+    water = water > a ? water : a;  // maximum (a is never NaN)
+  }
+  double a1 = water;
+  if (a1 < 1.0e-6) a1 = 1.0e-6;
+  if (a1 > 1.0 - 1.0e-6) a1 = 1.0 - 1.0e-6;
+  const bool post = x < ic.shock_x;
+  ic_store(G, e, U, c, a1, ic.rho_water, post ? ic.rho_post : ic.rho_air,
+           post ? ic.u_post : 0.0, 0.0, post ? ic.p_post : ic.p0);
+}
+
+__global__ void __launch_bounds__(256) k_ic_triple(Geo G, Eos e, State U, TripleIC ic) {
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long n = (long long)(G.nx + 2 * G.g) * G.ld;
+  if (k >= n) return;
+  const int i = (int)(k / G.ld), j = (int)(k % G.ld);
+  const size_t c = (size_t)k;
+  if (i < G.g || i >= G.g + G.nx || j < G.g || j >= G.g + G.ny) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) U.f[m][c] = 0.0;
+    return;
+  }
+  const double x = ic.xmin + ((double)(i - G.g) + 0.5) * G.dx;
+  const double y = ic.ymin + ((double)(j - G.g) + 0.5) * G.dy;
+  const bool left = x < ic.x_split;
+  const bool top = !left && y >= ic.y_split;
+  const double a1 = (left || top) ? 1.0 - 1.0e-6 : 1.0e-6;
+  const double rho = left ? ic.rho_left : (top ? ic.rho_top : ic.rho_bottom);
+  const double p = left ? ic.p_left : (top ? ic.p_top : ic.p_bottom);
+  ic_store(G, e, U, c, a1, rho, rho, 0.0, 0.0, p);
+}
+
+// ========================================================================
 // launch helpers (called by the C-ABI layer)
 // ========================================================================
 typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
@@ -1283,6 +1360,22 @@ hipError_t wamp_launch_bc(const Geo* G, const State* U, double* up, double* vp, 
   const int n = (G->nx + 2 * G->g) * (G->ny + 2 * G->g);
   hipLaunchKernelGGL(k_bc, dim3((n + STHREADS - 1) / STHREADS), dim3(STHREADS), 0, st, *G, *U, up,
                      vp, pp);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_ic_droplets(const Geo* G, const Eos* e, const State* U, const DropletIC* ic,
+                                  hipStream_t st) {
+  const long long n = (long long)(G->nx + 2 * G->g) * G->ld;
+  hipLaunchKernelGGL(k_ic_droplets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *G, *e,
+                     *U, *ic);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_ic_triple(const Geo* G, const Eos* e, const State* U, const TripleIC* ic,
+                                 hipStream_t st) {
+  const long long n = (long long)(G->nx + 2 * G->g) * G->ld;
+  hipLaunchKernelGGL(k_ic_triple, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, *G, *e,
+                     *U, *ic);
   return hipGetLastError();
 }
 

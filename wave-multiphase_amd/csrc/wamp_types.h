@@ -41,6 +41,24 @@ struct CaseDesc {
   double cfl;
 };
 
+// Initial conditions built on the device (cases of the measured workloads).
+// Planar shock at shock_x moving +x into air at rest holding water droplets
+// (xc, yc, r, r_cavity) smeared over `width` with a tanh profile.
+struct DropletIC {
+  double xmin, ymin;
+  double shock_x, rho_post, p_post, u_post;
+  double rho_water, rho_air, p0;
+  double width;
+  const double* drops;  // n_drops x 4, device memory
+  int n_drops;
+};
+// Three constant states split at x_split and (for x >= x_split) y_split:
+// left and top in phase 1, bottom in phase 2 (the gas-gas triple point).
+struct TripleIC {
+  double xmin, ymin, x_split, y_split;
+  double rho_left, p_left, rho_top, p_top, rho_bottom, p_bottom;
+};
+
 // Per-slot device scalars of a batch.
 struct BatchScal {
   double* dt;

@@ -33,6 +33,18 @@ class WampScheme(C.Structure):
                 ("face_pp", C.c_int32), ("repair", C.c_int32), ("cfl", C.c_double)]
 
 
+class WampDropletIC(C.Structure):
+    _fields_ = [("xmin", C.c_double), ("ymin", C.c_double), ("shock_x", C.c_double),
+                ("rho_post", C.c_double), ("p_post", C.c_double), ("u_post", C.c_double),
+                ("rho_water", C.c_double), ("rho_air", C.c_double), ("p0", C.c_double),
+                ("width", C.c_double), ("drops", C.c_void_p), ("n_drops", C.c_int32)]
+
+
+class WampTripleIC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("xmin", "ymin", "x_split", "y_split", "rho_left",
+                                           "p_left", "rho_top", "p_top", "rho_bottom", "p_bottom")]
+
+
 PtrArr6 = C.c_void_p * 6
 PtrArr3 = C.c_void_p * 3
 PtrArr28 = C.c_void_p * 28
@@ -54,6 +66,10 @@ SIGNATURES = [
                                     C.c_double, _VP, _VP, _VP]),
     ("wamp_get_dt", C.c_int, [_VP, PtrArr6, _VP, _VP]),
     ("wamp_step", C.c_int, [_VP, PtrArr6, _VP, _VP, _VP]),
+    ("wamp_init_droplets", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos),
+                                     C.POINTER(WampDropletIC), PtrArr6, _VP]),
+    ("wamp_init_triple_point", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos),
+                                         C.POINTER(WampTripleIC), PtrArr6, _VP]),
     ("wamp_batch_create", C.c_int, [C.POINTER(WampScheme), C.c_int32, C.c_int32, C.POINTER(_VP)]),
     ("wamp_batch_destroy", C.c_int, [_VP]),
     ("wamp_batch_set_case", C.c_int, [_VP, C.c_int32, C.POINTER(WampGrid), C.POINTER(WampEos),

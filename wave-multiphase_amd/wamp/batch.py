@@ -83,8 +83,15 @@ class Batch:
         g = cfg.grid
         if cfg.precision != self.precision or cfg.scheme != self.scheme:
             raise ValueError("a batch holds one scheme and one precision")
-        U = torch.zeros((6, g.nxg, g.nyg), dtype=torch.float64, device=self.device)
-        U.copy_(torch.from_numpy(np.ascontiguousarray(U0, dtype=np.float64)))
+        if isinstance(U0, torch.Tensor) and U0.device == self.device and \
+                tuple(U0.shape) == (6, g.nxg, g.nyg) and U0.dtype == torch.float64 and U0.is_contiguous():
+            U = U0  # a device state built for this slot (cases.*_device): kept, not copied
+        else:
+            U = torch.zeros((6, g.nxg, g.nyg), dtype=torch.float64, device=self.device)
+            if isinstance(U0, torch.Tensor):
+                U.copy_(U0[:, :, : g.nyg])
+            else:
+                U.copy_(torch.from_numpy(np.ascontiguousarray(U0, dtype=np.float64)))
         grid = _lib.WampGrid(g.nx, g.ny, g.g, g.nyg, g.dx, g.dy, (C.c_int32 * 4)(*g.bc))
         eos = _lib.WampEos(*cfg.eos.as_tuple())
         ptrs = _lib.ptrs6([U[m].data_ptr() for m in range(6)])

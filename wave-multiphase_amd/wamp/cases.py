@@ -163,10 +163,15 @@ def batch_case(rng, scheme=SCHEME_MP5, grids=BATCH_GRIDS):
     return cfg, U
 
 
+def triple_point_grid(nx, ny) -> Grid:
+    """[0,7]x[0,3], all walls slip."""
+    return Grid(nx, ny, 0.0, 7.0, 0.0, 3.0,
+                bc=(BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL))
+
+
 def triple_point(nx=448, ny=192, scheme=SCHEME_MP5):
     """Gas-gas triple point on [0,7]x[0,3], all walls slip."""
-    grid = Grid(nx, ny, 0.0, 7.0, 0.0, 3.0,
-                bc=(BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL, BC_SLIP_WALL))
+    grid = triple_point_grid(nx, ny)
     x = grid.x_centres()[:, None] * np.ones((1, grid.nyg))
     y = grid.y_centres()[None, :] * np.ones((grid.nxg, 1))
     left = x < 1.0
@@ -180,9 +185,72 @@ def triple_point(nx=448, ny=192, scheme=SCHEME_MP5):
     return cfg, _zero_ghosts(U, grid)
 
 
+# ---------------------------------------------------------------------------
+# The same cases built in device memory (wamp_init_droplets / _triple_point)
+# ---------------------------------------------------------------------------
+
+def _abi_grid_eos(grid: Grid, eos: EOS, ld: int):
+    import ctypes as C
+
+    from . import _lib
+
+    return (_lib.WampGrid(grid.nx, grid.ny, grid.g, ld, grid.dx, grid.dy, (C.c_int32 * 4)(*grid.bc)),
+            _lib.WampEos(*eos.as_tuple()))
+
+
+def shock_droplets_device(grid: Grid, eos: EOS, mach: float, shock_x: float, droplets,
+                          rho_water: float = 1000.0, rho_air: float = 1.0, p0: float = 1.0,
+                          smear_cells: float = 1.5, device: str = "cuda"):
+    """Device twin of shock_droplets: a (6, NXG, NYG) FP64 tensor built by one kernel.
+    Equal to the host state up to the last-ulp differences of tanh between the
+    device and host math libraries."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib
+
+    L = _lib.load()
+    ld = grid.nyg
+    U = torch.empty((6, grid.nxg, ld), dtype=torch.float64, device=device)
+    drops = np.asarray(droplets, dtype=np.float64).reshape(-1, 4)
+    d_dev = torch.from_numpy(drops).to(device) if len(drops) else None
+    rho_s, p_s, u_s = rankine_hugoniot(mach, eos.gamma2, rho_air, p0)
+    ic = _lib.WampDropletIC(grid.xmin, grid.ymin, shock_x, rho_s, p_s, u_s, rho_water, rho_air, p0,
+                            smear_cells * grid.dx, d_dev.data_ptr() if d_dev is not None else None,
+                            len(drops))
+    g_s, e_s = _abi_grid_eos(grid, eos, ld)
+    stream = torch.cuda.current_stream(U.device).cuda_stream
+    _lib.check(L.wamp_init_droplets(C.byref(g_s), C.byref(e_s), C.byref(ic),
+                                    _lib.ptrs6([U[m].data_ptr() for m in range(6)]), stream),
+               "wamp_init_droplets")
+    return U  # d_dev is freed in stream order after the kernel
+
+
+def triple_point_device(nx=448, ny=192, scheme=SCHEME_MP5, device: str = "cuda"):
+    """Device twin of triple_point (bitwise the same state)."""
+    import ctypes as C
+
+    import torch
+
+    from . import _lib
+
+    L = _lib.load()
+    grid = triple_point_grid(nx, ny)
+    U = torch.empty((6, grid.nxg, grid.nyg), dtype=torch.float64, device=device)
+    ic = _lib.WampTripleIC(grid.xmin, grid.ymin, 1.0, 1.5, 1.0, 1.0, 0.125, 0.1, 1.0, 0.1)
+    g_s, e_s = _abi_grid_eos(grid, TRIPLE_EOS, grid.nyg)
+    stream = torch.cuda.current_stream(U.device).cuda_stream
+    _lib.check(L.wamp_init_triple_point(C.byref(g_s), C.byref(e_s), C.byref(ic),
+                                        _lib.ptrs6([U[m].data_ptr() for m in range(6)]), stream),
+               "wamp_init_triple_point")
+    cfg = CaseConfig(grid, TRIPLE_EOS, scheme, name=f"triple_point_{nx}x{ny}")
+    return cfg, U
+
+
 __all__ = [
     "rankine_hugoniot", "conserved_from_prims", "shock_droplets", "shock_cylinder",
     "mach10_cylinder", "multi_droplet", "seeded_droplets", "batch_case", "triple_point",
     "WATER_ND", "TRIPLE_EOS", "BATCH_GRIDS", "SCHEME_SC", "SCHEME_M10", "SCHEME_MP5",
-    "BC_ZERO_GRADIENT",
+    "BC_ZERO_GRADIENT", "shock_droplets_device", "triple_point_device",
 ]

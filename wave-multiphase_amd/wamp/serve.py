@@ -33,11 +33,18 @@ PLANES_PER_CASE = 46  # 6 state planes + 28 face planes + 2 x 6 stage planes
 
 @dataclass(order=True)
 class Request:
+    """A solve request: case, initial state, end time, arrival time.  The
+    initial state is a host array (U0) or, with U0 None, built on the device
+    when the request is admitted (build(), e.g. cases.*_device)."""
     arrival: float
     rid: int = field(compare=False)
     cfg: CaseConfig = field(compare=False)
     U0: np.ndarray = field(compare=False, repr=False)
     t_end: float = field(compare=False)
+    build: object = field(default=None, compare=False, repr=False)
+
+    def state0(self):
+        return self.U0 if self.U0 is not None else self.build()
 
     @property
     def nbytes(self) -> int:
@@ -114,7 +121,7 @@ class Server:
                 break  # first-come first-served: wait for memory
             heapq.heappop(self.queue)
             slot = self.free.pop(0)
-            self.batch.set_case(slot, req.cfg, req.U0, t0=0.0, t_end=req.t_end)
+            self.batch.set_case(slot, req.cfg, req.state0(), t0=0.0, t_end=req.t_end)
             self.used += req.nbytes
             self.slot_req[slot] = (req, now, self.rounds)
             new.append(slot)
@@ -159,9 +166,12 @@ T_END_DROPLET = 0.05
 T_END_TRIPLE = 0.5
 
 
-def make_request(rng, rid: int, arrival: float, scheme=SCHEME_MP5, precision=PREC_MIXED):
+def make_request(rng, rid: int, arrival: float, scheme=SCHEME_MP5, precision=PREC_MIXED,
+                 device_ic: bool = False):
     """Log-uniform NX in [128, 2048] (multiple of 32), NY = clamp(NX/2, 128, 1024);
-    half shock-droplet cases (droplets ~ area, at least one), half triple points."""
+    half shock-droplet cases (droplets ~ area, at least one), half triple points.
+    device_ic: the request carries its case recipe and its initial state is
+    built in device memory at admission instead of on the host now."""
     nx = int(round(math.exp(rng.uniform(math.log(128), math.log(2048))) / 32.0)) * 32
     nx = min(max(nx, 128), 2048)
     ny = min(max(int(round(nx / 2 / 32.0)) * 32, 128), 1024)
@@ -172,13 +182,20 @@ def make_request(rng, rid: int, arrival: float, scheme=SCHEME_MP5, precision=PRE
         n = max(1, int(round(40 * nx * ny / (4096 * 2048))))
         drops = cases.seeded_droplets(rng, n, 0.05, 0.15, 0.9, 3.9, 0.05, 1.95, 4.0 * grid.dx,
                                       n_cavity=n // 4)
-        U0 = cases.shock_droplets(grid, cases.WATER_ND, 2.4, 0.8, drops)
         cfg = CaseConfig(grid, cases.WATER_ND, scheme, precision, name=f"droplets_{nx}x{ny}")
         t_end = T_END_DROPLET
+        if device_ic:
+            return Request(arrival, rid, cfg, None, t_end,
+                           build=lambda: cases.shock_droplets_device(grid, cases.WATER_ND, 2.4, 0.8, drops))
+        U0 = cases.shock_droplets(grid, cases.WATER_ND, 2.4, 0.8, drops)
     else:
-        cfg, U0 = cases.triple_point(nx, ny, scheme=scheme)
-        cfg = CaseConfig(cfg.grid, cfg.eos, scheme, precision, name=cfg.name)
         t_end = T_END_TRIPLE
+        grid = cases.triple_point_grid(nx, ny)
+        cfg = CaseConfig(grid, cases.TRIPLE_EOS, scheme, precision, name=f"triple_point_{nx}x{ny}")
+        if device_ic:
+            return Request(arrival, rid, cfg, None, t_end,
+                           build=lambda: cases.triple_point_device(nx, ny, scheme=scheme)[1])
+        U0 = cases.triple_point(nx, ny, scheme=scheme)[1]
     return Request(arrival, rid, cfg, U0, t_end)
 
 
@@ -199,7 +216,7 @@ def with_arrivals(reqs, rate: float | None, base: float = 0.0, precision: int | 
     for r in reqs:
         cfg = r.cfg if precision is None else dataclasses.replace(r.cfg, precision=precision)
         arrival = base if rate is None else base + r.arrival / rate
-        out.append(Request(arrival, r.rid, cfg, r.U0, r.t_end))
+        out.append(dataclasses.replace(r, arrival=arrival, cfg=cfg))
     return out
 
 
@@ -233,7 +250,7 @@ def serve_plain(reqs, clock=time.perf_counter, keep=None, log=None):
             time.sleep(r.arrival - now)
         start = clock() - t0
         s = Solver(r.cfg)
-        s.set_state(r.U0)
+        s.set_state(r.state0())
         s.prepare()
         s.advance_to(r.t_end, log=None)
         U = s.state()
@@ -256,7 +273,7 @@ def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int 
                  round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 16))
     base = srv.clock()
     for r in reqs:
-        srv.submit(Request(base + r.arrival, r.rid, r.cfg, r.U0, r.t_end))
+        srv.submit(dataclasses.replace(r, arrival=base + r.arrival))
     elapsed = srv.run(log=log)
     for r in srv.results:
         r.arrival -= base

@@ -72,13 +72,17 @@ class Solver:
         return int(self.L.wamp_ctx_scratch_bytes(self.ctx))
 
     # -- state --------------------------------------------------------------
-    def set_state(self, U: np.ndarray):
-        """Upload a (6, NXG, NYG) conserved state (host FP64)."""
+    def set_state(self, U):
+        """Set the (6, NXG, NYG) conserved state: a host FP64 array (uploaded) or a
+        device tensor (copied on the device, e.g. from cases.*_device)."""
         g = self.cfg.grid
-        if U.shape != (6, g.nxg, g.nyg):
-            raise ValueError(f"state shape {U.shape} != {(6, g.nxg, g.nyg)}")
+        if tuple(U.shape[:2]) != (6, g.nxg) or U.shape[2] < g.nyg:
+            raise ValueError(f"state shape {tuple(U.shape)} != {(6, g.nxg, g.nyg)}")
         torch = _torch()
-        self.U[:, :, : g.nyg].copy_(torch.from_numpy(np.ascontiguousarray(U, dtype=np.float64)))
+        if isinstance(U, torch.Tensor):
+            self.U[:, :, : g.nyg].copy_(U[:, :, : g.nyg])
+        else:
+            self.U[:, :, : g.nyg].copy_(torch.from_numpy(np.ascontiguousarray(U, dtype=np.float64)))
 
     def state(self) -> np.ndarray:
         g = self.cfg.grid
