@@ -70,11 +70,15 @@ def test_cli_run_and_restart(gpu_ok, tmp_path):
     assert files[0] == "mp5_000000.npz" and len(files) == 3, files
     mid = [f for f in files if f != "mp5_000000.npz"][0]
     r = subprocess.run(base + ["--t-end", "0.02", "--restart", str(tmp_path / "a" / mid),
-                               "--out", str(tmp_path / "b")], capture_output=True, text=True, timeout=300)
+                               "--out", str(tmp_path / "b"), "--png"], capture_output=True, text=True,
+                       timeout=300)
     assert r.returncode == 0, r.stderr
     assert "Restart:" in r.stdout
+    pngs = [f for f in os.listdir(tmp_path / "b") if f.endswith(".png")]
+    assert len(pngs) == 1 and pngs[0].startswith("mp5_")
+    assert open(tmp_path / "b" / pngs[0], "rb").read(8) == b"\x89PNG\r\n\x1a\n"
     end_a = np.load(tmp_path / "a" / files[-1])
-    end_b = np.load(tmp_path / "b" / sorted(os.listdir(tmp_path / "b"))[-1])
+    end_b = np.load(tmp_path / "b" / sorted(f for f in os.listdir(tmp_path / "b") if f.endswith(".npz"))[-1])
     assert float(end_a["time"]) == pytest.approx(0.02) and float(end_b["time"]) == pytest.approx(0.02)
     # the restart recomputes E from p, so it is not bit-exact (SC:1811-1813)
     for k in ("rho", "u", "p", "alpha1"):

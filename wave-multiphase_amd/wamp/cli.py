@@ -4,7 +4,7 @@
         [--restart-time T] [--restart-step N]
         [--case shock_cylinder|mach10_cylinder|multi_droplet|triple_point]
         [--nx NX --ny NY] [--scheme sc|m10|mp5] [--t-end T] [--save-times T1,T2,...]
-        [--max-steps N] [--out DIR] [--prefix NAME]
+        [--max-steps N] [--out DIR] [--prefix NAME] [--png]
 
 The reference's flags keep their meaning: --cuda selects the GPU (the
 default), --restart/--restart-time/--restart-step restart from an npz file.
@@ -53,6 +53,9 @@ def parse(argv=None):
     ap.add_argument("--out", type=str, default=".")
     ap.add_argument("--prefix", type=str, default=None)
     ap.add_argument("--log-every", type=int, default=200)
+    ap.add_argument("--png", action="store_true",
+                    help="also write {prefix}_{step:06d}.png (density gradient + alpha1) with the "
+                         "final output, as SC:2080")
     args, _ = ap.parse_known_args(argv)
     return args
 
@@ -98,6 +101,10 @@ def main(argv=None) -> int:
     t0 = _time.time()
     s.advance_to(t_end, saves, on_save=save, log_every=args.log_every, max_steps=args.max_steps)
     save(s)
+    if args.png:
+        path = os.path.join(args.out, io.output_name(prefix, s.nstep, "png"))
+        io.save_png(path, g, s.state())
+        print(f"  saved {path}")
     print(f"Done. Total steps={s.nstep}, t={s.time:.4e}  wall={_time.time() - t0:.1f}s")
     s.close()
     return 0

@@ -5,6 +5,8 @@ save_npz            SC:1712-1726 (keys time, x, y, rho, ar1, ar2, u, v, p, alpha
 infer_restart_step  SC:1773-1776 (digits after the last '_' of the file stem)
 load_restart        SC:1779-1821 (required keys, shape check, alpha clipped to
                     [1e-8, 1-1e-8], E rebuilt from p)
+save_png            SC:1728-1769 (density-gradient and alpha1 panels; written
+                    with numpy and zlib, no plotting library)
 
 The reference rebuilds E with its hard-coded water/air constants
 (SC:1811-1812), which equal its module EOS (SC:40-43); here E is rebuilt with
@@ -68,3 +70,61 @@ def load_restart(path: str, cfg: CaseConfig):
     for m, a in enumerate((f["ar1"], f["ar2"], ru, rv, E, al)):
         U[m][si, sj] = a
     return U, t
+
+
+# ---------------------------------------------------------------------------
+# Picture of a state: save_png SC:1728-1769 without matplotlib
+# ---------------------------------------------------------------------------
+def _png_bytes(rgb: np.ndarray) -> bytes:
+    """8-bit RGB image (H, W, 3) as a PNG file (one IDAT, no filtering)."""
+    import struct
+    import zlib
+
+    h, w, _ = rgb.shape
+    raw = b"".join(b"\x00" + rgb[r].astype(np.uint8).tobytes() for r in range(h))
+
+    def chunk(kind: bytes, data: bytes) -> bytes:
+        return (struct.pack(">I", len(data)) + kind + data
+                + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF))
+
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(raw, 6))
+            + chunk(b"IEND", b""))
+
+
+def _blues(t: np.ndarray) -> np.ndarray:
+    """t in [0, 1] -> RGB from near-white to dark blue (the reference's Blues map ends)."""
+    lo = np.array([247.0, 251.0, 255.0])
+    hi = np.array([8.0, 48.0, 107.0])
+    t = np.clip(t, 0.0, 1.0)[..., None]
+    return lo + (hi - lo) * t
+
+
+def save_png(path: str, grid: Grid, U: np.ndarray, vmin: float | None = None,
+             vmax: float | None = None) -> str:
+    """Two panels, as SC:1745-1769: left the density-gradient magnitude
+    (np.gradient central differences, SC:1735-1737) on a white-to-blue scale
+    with the alpha1 = 0.5 contour in red; right alpha1 on a white-to-black
+    scale.  Image rows are j, columns i (the transposed field, SC:1740); the
+    gradient scale is [vmin, vmax] (default: 0 to the field's maximum)."""
+    si, sj = grid.interior()
+    rho = (U[0] + U[1])[si, sj]
+    al = U[5][si, sj]
+    xg = grid.x_centres()[si]
+    yg = grid.y_centres()[sj]
+    grad = np.sqrt(np.gradient(rho, xg, axis=0) ** 2 + np.gradient(rho, yg, axis=1) ** 2)
+    lo = 0.0 if vmin is None else vmin
+    hi = float(grad.max()) if vmax is None else vmax
+    left = _blues((grad.T - lo) / (hi - lo if hi > lo else 1.0))
+    # alpha1 = 0.5 contour: cells whose alpha1 - 0.5 changes sign towards a neighbour
+    s = np.sign(al - 0.5)
+    edge = np.zeros_like(al, dtype=bool)
+    edge[:-1, :] |= s[:-1, :] != s[1:, :]
+    edge[:, :-1] |= s[:, :-1] != s[:, 1:]
+    left[edge.T] = (220.0, 20.0, 20.0)
+    right = np.repeat((255.0 * (1.0 - np.clip(al.T, 0.0, 1.0)))[..., None], 3, axis=2)
+    gap = np.full((al.shape[1], max(2, al.shape[0] // 64), 3), 255.0)
+    img = np.concatenate([left, gap, right], axis=1)
+    with open(path, "wb") as fh:
+        fh.write(_png_bytes(np.rint(img)))
+    return path
