@@ -285,3 +285,49 @@ def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int 
     results = srv.results
     srv.close()
     return results, st
+
+
+def sweep(n: int = 200, loads=(0.3, 0.5, 0.7, 0.9, 1.2), seed: int = 5, log=None) -> dict:
+    """Latency against offered load: the server's capacity from a saturated run
+    (all requests queued at t=0), then Poisson arrivals at each load x capacity.
+    Initial states are built on the device at admission."""
+    reqs = poisson_workload(n, 1.0, seed, device_ic=True)
+    warm = [r for r in reqs if r.cfg.grid.nx <= 192][:2]
+    serve_batched(with_arrivals(warm, None))
+    _, sat = serve_batched(with_arrivals(reqs, None))
+    cap = sat["completed"] / sat["elapsed_s"]
+    if log:
+        log(f"capacity {cap:.3f} requests/s ({sat['cases_per_hour']:.0f} cases/hour)")
+    rows = []
+    for load in loads:
+        rate = load * cap
+        _, st = serve_batched(with_arrivals(reqs, rate))
+        st.update(load=load, rate_per_s=rate)
+        rows.append(st)
+        if log:
+            log(f"load {load:.2f}: p50 {st['p50_latency_s']:.3f} s  p99 {st['p99_latency_s']:.3f} s  "
+                f"{st['cases_per_hour']:.0f} cases/hour")
+    return {"requests": n, "seed": seed, "capacity_per_s": cap, "saturated": sat, "loads": rows}
+
+
+def main(argv=None) -> int:
+    """python -m wamp.serve [--requests N] [--loads 0.3,0.5,...]: the load sweep as JSON."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--loads", type=str, default="0.3,0.5,0.7,0.9,1.2")
+    ap.add_argument("--seed", type=int, default=5)
+    args = ap.parse_args(argv)
+    loads = [float(x) for x in args.loads.split(",") if x.strip()]
+    out = sweep(args.requests, loads, args.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())
