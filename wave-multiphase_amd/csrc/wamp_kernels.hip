@@ -723,25 +723,31 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       yc[1] = yok ? F[8][c] : 0.0;
       yc[2] = yok ? F[21][c] : 0.0;
       yc[3] = yok ? F[22][c] : 0.0;
+      // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
+      double yd0[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) lds(LC_Y01 + 4 * s3f + q) = yc[q];
-      double u01[2] = {0.0, 0.0};
+      for (int m = 0; m < 2; ++m) {
+        yd0[m] = ok ? F[7 + m][c - 1] : 0.0;
+        yd0[2 + m] = ok ? F[21 + m][c - 1] : 0.0;
+      }
+      double u01[2];
+      u01[0] = ok ? U.f[0][c] : 0.0;
+      u01[1] = ok ? U.f[1][c] : 0.0;
       if (ok) {
         double l[6], h[6];
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
-          // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
-          double yuh = yc[m], ydh = F[7 + m][c - 1];
-          double yul = yc[2 + m], ydl = F[21 + m][c - 1];
-          double rh = (x01[2][m] - x01[1][m]) * inv_dx + (yuh - ydh) * inv_dy;
-          double rl = (x01[2][2 + m] - x01[1][2 + m]) * inv_dx + (yul - ydl) * inv_dy;
-          double u = U.f[m][c];
-          u01[m] = u;
-          l[m] = u - dt * rl;
-          h[m] = u - dt * rh;
+          double rh = (x01[2][m] - x01[1][m]) * inv_dx + (yc[m] - yd0[m]) * inv_dy;
+          double rl = (x01[2][2 + m] - x01[1][2 + m]) * inv_dx + (yc[2 + m] - yd0[2 + m]) * inv_dy;
+          l[m] = u01[m] - dt * rl;
+          h[m] = u01[m] - dt * rh;
         }
         th = theta_stage<0>(e, l, h);
       }
+      // carried values are stored once consumed: a store issued right after its
+      // load would wait for every load issued before it
+#pragma unroll
+      for (int q = 0; q < 4; ++q) lds(LC_Y01 + 4 * s3f + q) = yc[q];
       lds(LC_U01 + 2 * s4f) = u01[0];
       lds(LC_U01 + 2 * s4f + 1) = u01[1];
     }
@@ -755,25 +761,22 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const size_t c = at(r);
       const bool ok = row_in(r) && col_in;
       const bool yok = row_in(r) && yface_in;
-      double tf[4];
-      tf[0] = face_theta(t0[2], t0[3]);
-      tf[1] = face_theta(t0[3], t0[4]);
-      tf[2] = face_theta(lane_up(t0[3]), t0[3]);
-      tf[3] = face_theta(t0[3], lane_down(t0[3]));
       double yu[4];
       yu[0] = yok ? F[12][c] : 0.0;
       yu[1] = yok ? F[13][c] : 0.0;
       yu[2] = yok ? F[26][c] : 0.0;
       yu[3] = yok ? F[27][c] : 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) lds(LC_Y56 + 4 * s2m1 + q) = yu[q];
+      const double al = ok ? U.f[5][c] : 0.0;
+      double tf[4];
+      tf[0] = face_theta(t0[2], t0[3]);
+      tf[1] = face_theta(t0[3], t0[4]);
+      tf[2] = face_theta(lane_up(t0[3]), t0[3]);
+      tf[3] = face_theta(t0[3], lane_down(t0[3]));
       double yd[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) yd[q] = lane_up(yu[q]);
       double th1 = 1.0;
-      double al = 0.0;
       if (ok) {
-        al = U.f[5][c];
         double Hxp5 = blend3(x56[1][0], x56[1][2], &tf[1], 1);
         double Hxm5 = blend3(x56[0][0], x56[0][2], &tf[0], 1);
         double Hyp5 = blend3(yu[0], yu[2], &tf[3], 1);
@@ -791,6 +794,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
         h[5] = al - dt * rh;
         th1 = theta_stage<1>(e, l, h);
       }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) lds(LC_Y56 + 4 * s2m1 + q) = yu[q];
       lds(LC_U5 + s3m1) = al;
 #pragma unroll
       for (int k = 0; k < 3; ++k) t1[k] = t1[k + 1];
@@ -830,18 +835,18 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 #pragma unroll
       for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
       double th2 = 1.0;
-      double u234[3] = {0.0, 0.0, 0.0};
+      double u234[3];
+      u234[0] = ok ? U.f[2][c] : 0.0;
+      u234[1] = ok ? U.f[3][c] : 0.0;
+      u234[2] = ok ? U.f[4][c] : 0.0;
       if (ok) {
         double uc[6];
         uc[0] = lds(LC_U01 + 2 * s4m2);
         uc[1] = lds(LC_U01 + 2 * s4m2 + 1);
-        uc[2] = U.f[2][c];
-        uc[3] = U.f[3][c];
-        uc[4] = U.f[4][c];
+        uc[2] = u234[0];
+        uc[3] = u234[1];
+        uc[4] = u234[2];
         uc[5] = lds(LC_U5 + s3m2);
-        u234[0] = uc[2];
-        u234[1] = uc[3];
-        u234[2] = uc[4];
         const double al = uc[5];
         // component 6 (contact velocity) first: it enters component 5's residual
         const double s_hm = blend3(xa[0][6], xa[0][13], tf[0], 2);
@@ -896,10 +901,14 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
     {
       const int r = f - 3;
       const size_t c = at(r);
+      const bool upd = r >= i0 && r < i_end && out_lane;
+      double unv[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) unv[m] = upd ? Un.f[m][c] : 0.0;
       double yd[7];
 #pragma unroll
       for (int m = 0; m < 7; ++m) yd[m] = lane_up(F2yu[m]);
-      if (r >= i0 && r < i_end && out_lane) {
+      if (upd) {
         // row f-3 is the previous front's theta2 row (s2m1 is its U2..U4 slot)
         double uc[6];
         uc[0] = lds(LC_U01 + 2 * s4m3);
@@ -915,7 +924,7 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
           double res = (F2xp[m] - F2xm[m]) * inv_dx + (F2yu[m] - yd[m]) * inv_dy;
           if (m == 5)
             res = res - ap * (F2xp[6] - F2xm[6]) * inv_dx - ap * (F2yu[6] - yd[6]) * inv_dy;
-          un[m] = alpha * Un.f[m][c] + beta * (uc[m] - dt * res);
+          un[m] = alpha * unv[m] + beta * (uc[m] - dt * res);
         }
         if (repair) repair_cell(e, un);
         un[5] = clamp_al(un[5]);
