@@ -712,27 +712,57 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       }
     }
 
-    // ---------------- theta0 at row f (raw high-order fluxes) ----------------
-    double th = 1.0;
+    // ---- the front's remaining global loads, all issued before its arithmetic so
+    //      that they are in flight together (each stage below only waits for its own)
+    const bool upd = f - 3 >= i0 && f - 3 < i_end && out_lane;  // row f-3 is output
+    double yc[4], yd0[4], u01[2];  // theta0, row f
     {
       const size_t c = at(f);
       const bool ok = row_in(f) && col_in;
       const bool yok = row_in(f) && yface_in;
-      double yc[4];
       yc[0] = yok ? F[7][c] : 0.0;
       yc[1] = yok ? F[8][c] : 0.0;
       yc[2] = yok ? F[21][c] : 0.0;
       yc[3] = yok ? F[22][c] : 0.0;
       // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
-      double yd0[4];
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
         yd0[m] = ok ? F[7 + m][c - 1] : 0.0;
         yd0[2 + m] = ok ? F[21 + m][c - 1] : 0.0;
       }
-      double u01[2];
       u01[0] = ok ? U.f[0][c] : 0.0;
       u01[1] = ok ? U.f[1][c] : 0.0;
+    }
+    double yu[4], al1;  // theta1, row f-1
+    {
+      const size_t c = at(f - 1);
+      const bool ok = row_in(f - 1) && col_in;
+      const bool yok = row_in(f - 1) && yface_in;
+      yu[0] = yok ? F[12][c] : 0.0;
+      yu[1] = yok ? F[13][c] : 0.0;
+      yu[2] = yok ? F[26][c] : 0.0;
+      yu[3] = yok ? F[27][c] : 0.0;
+      al1 = ok ? U.f[5][c] : 0.0;
+    }
+    double yn24[6], u234[3];  // theta2, row f-2: y-face components 2..4 (H, L), U2..U4
+    {
+      const size_t c = at(f - 2);
+      const bool ok = row_in(f - 2) && col_in;
+      const bool yok = row_in(f - 2) && yface_in;
+#pragma unroll
+      for (int m = 2; m < 5; ++m) {
+        yn24[m - 2] = yok ? F[7 + m][c] : 0.0;
+        yn24[m + 1] = yok ? F[21 + m][c] : 0.0;
+      }
+      u234[0] = ok ? U.f[2][c] : 0.0;
+      u234[1] = ok ? U.f[3][c] : 0.0;
+      u234[2] = ok ? U.f[4][c] : 0.0;
+    }
+
+    // ---------------- theta0 at row f (raw high-order fluxes) ----------------
+    double th = 1.0;
+    {
+      const bool ok = row_in(f) && col_in;
       if (ok) {
         double l[6], h[6];
 #pragma unroll
@@ -758,15 +788,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
     // ---------------- theta1 at row f-1 (fluxes blended once) ----------------
     {
       const int r = f - 1;
-      const size_t c = at(r);
       const bool ok = row_in(r) && col_in;
-      const bool yok = row_in(r) && yface_in;
-      double yu[4];
-      yu[0] = yok ? F[12][c] : 0.0;
-      yu[1] = yok ? F[13][c] : 0.0;
-      yu[2] = yok ? F[26][c] : 0.0;
-      yu[3] = yok ? F[27][c] : 0.0;
-      const double al = ok ? U.f[5][c] : 0.0;
+      const double al = al1;
       double tf[4];
       tf[0] = face_theta(t0[2], t0[3]);
       tf[1] = face_theta(t0[3], t0[4]);
@@ -806,9 +829,7 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
     double yn[14];  // y-faces above row f-2 (H0..6, L0..6)
     {
       const int r = f - 2;
-      const size_t c = at(r);
       const bool ok = row_in(r) && col_in;
-      const bool yok = row_in(r) && yface_in;
       yn[0] = lds(LC_Y01 + 4 * s3m2 + 0);
       yn[1] = lds(LC_Y01 + 4 * s3m2 + 1);
       yn[7] = lds(LC_Y01 + 4 * s3m2 + 2);
@@ -819,8 +840,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       yn[13] = lds(LC_Y56 + 4 * s2m2 + 3);
 #pragma unroll
       for (int m = 2; m < 5; ++m) {
-        yn[m] = yok ? F[7 + m][c] : 0.0;
-        yn[7 + m] = yok ? F[21 + m][c] : 0.0;
+        yn[m] = yn24[m - 2];
+        yn[7 + m] = yn24[m + 1];
       }
       double tf[4][2];
       tf[0][0] = face_theta(t0[1], t0[2]);
@@ -835,10 +856,6 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 #pragma unroll
       for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
       double th2 = 1.0;
-      double u234[3];
-      u234[0] = ok ? U.f[2][c] : 0.0;
-      u234[1] = ok ? U.f[3][c] : 0.0;
-      u234[2] = ok ? U.f[4][c] : 0.0;
       if (ok) {
         double uc[6];
         uc[0] = lds(LC_U01 + 2 * s4m2);
@@ -901,7 +918,6 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
     {
       const int r = f - 3;
       const size_t c = at(r);
-      const bool upd = r >= i0 && r < i_end && out_lane;
       double unv[6];
 #pragma unroll
       for (int m = 0; m < 6; ++m) unv[m] = upd ? Un.f[m][c] : 0.0;
