@@ -69,3 +69,31 @@ def test_full_size_symmetry_and_admissibility(gpu_ok, case):
     ar1, ar2, al = S[0][si, sj], S[1][si, sj], S[5][si, sj]
     assert ar1.min() > 0.0 and ar2.min() > 0.0
     assert al.min() >= 1.0e-8 and al.max() <= 1.0 - 1.0e-8
+
+
+@pytest.mark.skipif(not os.environ.get("WAMP_FULLSIZE_STEPS"),
+                    reason="set WAMP_FULLSIZE_STEPS=N to compare N full-size steps with the oracle "
+                           "(the oracle takes ~2.3 s per step on 16 cores)")
+def test_full_size_many_steps_match_oracle(gpu_ok, case):
+    from oracle.oracle import Oracle
+    from wamp.solver import Solver
+
+    cfg, U0 = case
+    n = int(os.environ["WAMP_FULLSIZE_STEPS"])
+    orc = Oracle(cfg, threads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    orc.set_state(U0)
+    orc.prepare()
+    U = orc.state()
+    dts_ref = orc.run(n)
+    ref = orc.state()
+    del orc
+    s = Solver(cfg)
+    s.set_state(U)
+    dts = s.run(n)
+    got = s.state()
+    s.close()
+    si, sj = cfg.grid.interior()
+    errs = [rel_l1(got[m][si, sj], ref[m][si, sj]) for m in range(6)]
+    print(f"4096x2048, {n} steps:", ["%.2e" % e for e in errs],
+          "dt rel", float(np.abs(dts - dts_ref).max() / dts_ref.max()))
+    assert max(errs) <= 1e-9
