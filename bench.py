@@ -282,6 +282,20 @@ def measured_step_work(workload: str):
     return int(vals["fp64_flop_per_step"]), int(vals["valu_wave_instr_per_step"])
 
 
+def kernel_valu(workload: str, kernel: str, launch_ms: float) -> dict:
+    """The kernel's VALU issue and FP64 FLOP rates at its live launch time, from
+    the committed per-launch instruction counts (the kernel is VALU-bound; the
+    contract's roofline object is HBM-framed)."""
+    vals = _record(f"{kernel}_traffic_{workload}.txt")
+    if vals.get("workload") != workload or "valu_wave_instr" not in vals:
+        return {}
+    t = launch_ms / 1e3
+    valu, flop = int(vals["valu_wave_instr"]), int(vals["fp64_flop"])
+    return {"valu_issue_frac": round(valu * 4 / (VALU_SIMDS * CLOCK_HZ * t), 4),
+            "fp64_tflops": round(flop / t / 1e12, 3),
+            "fp64_frac": round(flop / t / (FP64_PEAK_TFLOPS * 1e12), 4)}
+
+
 def valu_fracs(workload: str, ms_per_step: float):
     """FP64 FLOP rate and VALU issue rate of the measured step against the
     chip's peaks (78.6 TFLOP/s FP64; 1024 SIMDs x 2.4 GHz / 4 cycles per wave
@@ -416,6 +430,7 @@ def main(argv=None):
             "traffic": measured_traffic(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces"),
             "launch_ms": round(faces_ms, 4),
             "launches": int(nl[0]),
+            **kernel_valu(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces", faces_ms),
         },
         "kernels_ms": {"k_faces": round(faces_ms, 4), "k_limit_update": round(lu_ms, 4)},
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),
