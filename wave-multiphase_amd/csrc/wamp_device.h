@@ -21,15 +21,38 @@
 
 #include "wamp_types.h"
 
-namespace wamp {
+// The FP64 kernels (wamp_kernels.hip) and the FP32-mixed face kernels
+// (wamp_faces_mixed.hip, defining WAMP_MIXED_TU) are separate translation
+// units whose device helpers differ in FP64 division; the mixed unit's
+// helpers live in the inline namespace wamp::mixed so the two sets never
+// share a symbol.
+#ifdef WAMP_MIXED_TU
+#define WAMP_DEV_BEGIN namespace wamp { inline namespace mixed {
+#define WAMP_DEV_END } }
+#else
+#define WAMP_DEV_BEGIN namespace wamp {
+#define WAMP_DEV_END }
+#endif
 
-
+WAMP_DEV_BEGIN
 
 // Warp builtin semantics: comparison selects, sign(0) = +1.
 template <typename T> __device__ __forceinline__ T wmin(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T wmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T wsign(T x) { return x < T(0) ? T(-1) : T(1); }
 template <typename T> __device__ __forceinline__ bool isnan_(T x) { return __builtin_isnan(x); }
+// FP32 (the mixed mode's face arithmetic only): the hardware v_min_f32 /
+// v_max_f32, one instruction instead of a compare and a select.  For
+// operands that are not NaN they return the same value as the comparison
+// selects (a pair of zeros may come back with the other sign).
+template <> __device__ __forceinline__ float wmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ __forceinline__ float wmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+// sign inside the minmod limiters: there every term is multiplied by the
+// smallest magnitude of its arguments, which is zero whenever an argument is
+// a zero of either sign, so copysign (one v_bfi) gives the same value as
+// wsign for FP32.
+template <typename T> __device__ __forceinline__ T msign(T x) { return wsign(x); }
+template <> __device__ __forceinline__ float msign<float>(float x) { return __builtin_copysignf(1.0f, x); }
 
 // Division, square root and tanh by arithmetic type.  FP64 keeps IEEE
 // correctly rounded operations (the parity path).  The FP32 face arithmetic
@@ -37,7 +60,24 @@ template <typename T> __device__ __forceinline__ bool isnan_(T x) { return __bui
 // v_sqrt_f32 and an exp-based tanh, about 1-2 ulp): on CDNA4 non-packed
 // FP32 issues at the FP64 rate, so the mixed mode gains only by dropping the
 // multi-instruction IEEE sequences.
-__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+// FP64 division.  The FP64 kernels keep the IEEE correctly rounded divide.
+// The mixed unit's remaining FP64 arithmetic (pressure recovery, indicators,
+// face positivity limiter, energy flux) takes the hardware reciprocal, one
+// Newton step and one residual correction (Markstein): within one ulp of the
+// quotient, six instructions instead of the eleven of the IEEE sequence.
+__device__ __forceinline__ double ddiv(double a, double b) {
+#ifdef WAMP_MIXED_TU
+  double y = __builtin_amdgcn_rcp(b);
+  const double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
+#else
+  return a / b;
+#endif
+}
+__device__ __forceinline__ double dv(double a, double b) { return ddiv(a, b); }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sq(double x) { return sqrt(x); }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
@@ -136,11 +176,11 @@ __device__ __forceinline__ double pressure(const Eos& e, double ar1, double ar2,
                                            double rv, double E, double a1, double& uu,
                                            double& vv) {
   double rho = ar1 + ar2;
-  uu = ru / rho;
-  vv = rv / rho;
+  uu = ddiv(ru, rho);
+  vv = ddiv(rv, rho);
   double gb, pb;
   mix_gb_pb(e, a1, gb, pb);
-  return (E - pb - 0.5 * rho * (uu * uu + vv * vv)) / gb;
+  return ddiv(E - pb - 0.5 * rho * (uu * uu + vv * vv), gb);
 }
 
 // ------------------------------------------------------- positivity --------
@@ -195,12 +235,12 @@ __device__ __forceinline__ double theta_stage(const Eos& e, const double* l, con
 // ------------------------------------------------- reconstruction ----------
 template <typename T>
 __device__ __forceinline__ T mm2(T x, T y) {  // SC:295-297
-  return T(0.5) * (wsign(x) + wsign(y)) * wmin(fabs(x), fabs(y));
+  return T(0.5) * (msign(x) + msign(y)) * wmin(fabs(x), fabs(y));
 }
 template <typename T>
 __device__ __forceinline__ T mm4(T a, T b, T c, T d) {  // SC:299-304
-  T s = wsign(a) + wsign(b);
-  return T(0.125) * s * fabs((wsign(a) + wsign(c)) * (wsign(a) + wsign(d))) *
+  T s = msign(a) + msign(b);
+  return T(0.125) * s * fabs((msign(a) + msign(c)) * (msign(a) + msign(d))) *
          wmin(wmin(fabs(a), fabs(b)), wmin(fabs(c), fabs(d)));
 }
 // cell_entropy SC:308-314
@@ -209,13 +249,13 @@ __device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, 
   if (rho < TINY) rho = TINY;
   double g, pi;
   mix_g_pi(e, a1, g, pi);
-  return p / pow(rho, g);
+  return ddiv(p, pow(rho, g));
 }
 // cator2_val SC:318-325
 __device__ __forceinline__ double cator2(double em2, double em1, double e0, double ep1, double ep2) {
   double aa = 0.25 * fabs(em2 - 4.0 * em1 + 3.0 * e0) + (13.0 / 12.0) * fabs(e0 - 2.0 * em1 + em2);
   double bb = 0.25 * fabs(3.0 * e0 - 4.0 * ep1 + ep2) + (13.0 / 12.0) * fabs(e0 - 2.0 * ep1 + ep2);
-  return (2.0 * aa * bb + YUXIN) / (aa * aa + bb * bb + YUXIN);
+  return ddiv(2.0 * aa * bb + YUXIN, aa * aa + bb * bb + YUXIN);
 }
 // thinc_ul SC:327-337; t1 = tanh(beta/2) is a per-launch constant
 template <typename T>
@@ -378,7 +418,7 @@ __device__ __forceinline__ double rho_ct2_sc(const Eos& e, const double* s) {
   double a1 = clamp01_pp(s[5]);
   double g, pi;
   mix_g_pi(e, a1, g, pi);
-  return (s[4] + pi) / (g - 1.0);
+  return ddiv(s[4] + pi, g - 1.0);
 }
 // wong_stage_lower_s6 SC:189-201 (in place on s)
 __device__ __forceinline__ void wong_lower(const double* b, double* s, double bv, double hv,
@@ -388,7 +428,7 @@ __device__ __forceinline__ void wong_lower(const double* b, double* s, double bv
     theta = 0.0;
   } else if (hv < floor_) {
     double denom = bv - hv;
-    theta = denom > TINY ? (bv - floor_) / denom : 0.0;
+    theta = denom > TINY ? ddiv(bv - floor_, denom) : 0.0;
   }
 #pragma unroll
   for (int m = 0; m < 6; ++m) s[m] = b[m] + theta * (s[m] - b[m]);
@@ -548,7 +588,7 @@ __device__ __forceinline__ void hllc_mixed(const Eos& e, const float* L, const f
       f[3] = (rhoL * vL * vL + pL) + SL * (rhoL * SP * ratio - rhoL * vL);
     }
     const double sl = SL, sp = SP, nl = nL, rl = rhoL, pl = pL;
-    const double EL_star = EL + (sp - nl) * (sp * rl + pl / (sl - nl));
+    const double EL_star = EL + (sp - nl) * (sp * rl + ddiv(pl, sl - nl));
     f[4] = (EL + pl) * nl + sl * (EL_star * (double)ratio - EL);
     f[5] = a1L * nL + SL * (a1L * ratio - a1L);
   } else if (SP <= 0.f && 0.f <= SR) {
@@ -563,7 +603,7 @@ __device__ __forceinline__ void hllc_mixed(const Eos& e, const float* L, const f
       f[3] = (rhoR * vR * vR + pR) + SR * (rhoR * SP * ratio - rhoR * vR);
     }
     const double sr = SR, sp = SP, nr = nR, rr = rhoR, pr = pR;
-    const double ER_star = ER + (sp - nr) * (sp * rr + pr / (sr - nr));
+    const double ER_star = ER + (sp - nr) * (sp * rr + ddiv(pr, sr - nr));
     f[4] = (ER + pr) * nr + sr * (ER_star * (double)ratio - ER);
     f[5] = a1R * nR + SR * (a1R * ratio - a1R);
   } else {
@@ -580,4 +620,4 @@ __device__ __forceinline__ void hllc_mixed(const Eos& e, const float* L, const f
   ustar = 0.5f * (1.f + sp_sign) * left_part + 0.5f * (1.f - sp_sign) * right_part;
 }
 
-}  // namespace wamp
+WAMP_DEV_END

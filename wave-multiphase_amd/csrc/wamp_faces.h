@@ -1,0 +1,375 @@
+// wamp_faces.h -- the fused face-flux kernel (k_faces, k_faces_batch) and its
+// tile routine, shared by two translation units: wamp_kernels.hip
+// instantiates the FP64 kernels (contraction off, the parity path) and
+// wamp_faces_mixed.hip the FP32-mixed ones (contraction on, fast FP64
+// division; namespace wamp::mixed).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wamp_device.h"
+#include "wamp_types.h"
+
+WAMP_DEV_BEGIN
+
+// ------------------------------------------------------------------------
+// Boundary fill by index mapping.  Ghost cell (i, j) holds the edge
+// interior cell: bc_x copies cell g / g+nx-1 into every x-ghost layer
+// (SC:1630-1646), then bc_y copies row g / g+ny-1 of the x-filled array
+// into every y-ghost layer (SC:1664-1682), so corner cells hold the corner
+// interior cell.  Slip walls negate the wall-normal momentum.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ size_t map_cell(const Geo& G, int i, int j, double& sru, double& srv) {
+  sru = 1.0;
+  srv = 1.0;
+  if (i < G.g) {
+    i = G.g;
+    if (G.bc[0]) sru = -1.0;
+  } else if (i >= G.g + G.nx) {
+    i = G.g + G.nx - 1;
+    if (G.bc[1]) sru = -1.0;
+  }
+  if (j < G.g) {
+    j = G.g;
+    if (G.bc[2]) srv = -1.0;
+  } else if (j >= G.g + G.ny) {
+    j = G.g + G.ny - 1;
+    if (G.bc[3]) srv = -1.0;
+  }
+  return (size_t)i * (size_t)G.ld + (size_t)j;
+}
+
+// ========================================================================
+// k_faces
+// ========================================================================
+// Tile of TI x TJ face positions (j contiguous), 256 threads.
+template <int TI_, int TJ_>
+struct FTile {
+  static constexpr int TI = TI_, TJ = TJ_;
+  static constexpr int RI = TI + 7, RJ = TJ + 7;  // LDS rectangle: cells i0-3 .. i0+TI+3
+  static constexpr int RN = RI * RJ;
+  static constexpr int CXN = (TI + 3) * TJ;       // x-indicator cells i0-1..i0+TI+1, j0..j0+TJ-1
+  static constexpr int CYN = TI * (TJ + 3);       // y-indicator cells i0..i0+TI-1, j0-1..j0+TJ+1
+  static constexpr int THREADS = TI * TJ;
+};
+using Tile8x32 = FTile<8, 32>;
+using Tile16x16 = FTile<16, 16>;
+constexpr int FTHREADS = 256;
+
+template <class TL>
+struct FaceSmem {
+  double q[6][TL::RN];  // ar1, ar2, ru, rv, p, a1
+  double ent[TL::RN];
+  double cx[TL::CXN], cy[TL::CYN];
+  double sx[TL::CXN], sy[TL::CYN];
+};
+
+// Ducros x pressure sensor (ducros_kernel SC:875-916) at LDS cell r.
+template <class TL>
+__device__ __forceinline__ double lds_vel(const FaceSmem<TL>& S, int r, int comp) {
+  return ddiv(S.q[comp][r], S.q[0][r] + S.q[1][r]);
+}
+template <class TL, int DIR>
+__device__ __forceinline__ double ducros_sigma(const FaceSmem<TL>& S, int r) {
+  constexpr int FRJ = TL::RJ;
+  double der_ux = 0.5 * (-lds_vel(S, r - FRJ, 2) + lds_vel(S, r + FRJ, 2));
+  double der_vx = 0.5 * (-lds_vel(S, r - FRJ, 3) + lds_vel(S, r + FRJ, 3));
+  double der_uy = 0.5 * (-lds_vel(S, r - 1, 2) + lds_vel(S, r + 1, 2));
+  double der_vy = 0.5 * (-lds_vel(S, r - 1, 3) + lds_vel(S, r + 1, 3));
+  double div_u = der_ux + der_vy;
+  double curl_u = der_vx - der_uy;
+  double ducros = ddiv(div_u * div_u, div_u * div_u + curl_u * curl_u + 1.0e-30);
+  const int s = DIR == 0 ? FRJ : 1;
+  const double* p = S.q[4];
+  double pm2 = p[r - 2 * s], pm1 = p[r - s], p0 = p[r], pp1 = p[r + s], pp2 = p[r + 2 * s];
+  double aa = fabs(-pm2 + 16.0 * pm1 - 30.0 * p0 + 16.0 * pp1 - pp2);
+  double bb = fabs(pm2 + 16.0 * pm1 + 30.0 * p0 + 16.0 * pp1 + pp2);
+  return ddiv(aa, bb + 1.0e-30) * ducros;
+}
+
+// High-order flux through one face whose left cell is LDS index r0; cells
+// along the normal are r0 + k*s.  ci / sg: indicator and Ducros sensor at
+// the cells -1..2.  Writes 6 fluxes + u* to out.
+template <class TL, int DIR, int BASE, int SHEAR, int FACEPP, typename T>
+__device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, int r0, const double* ci,
+                                          const double* sg, int vf_always, double t1d, double* out) {
+  constexpr int s = DIR == 0 ? TL::RJ : 1;
+  constexpr bool wide = BASE == 1 || SHEAR == 2;  // needs cells -2 and +3
+  constexpr int klo = wide ? 0 : 1, khi = wide ? 6 : 5;
+  const T t1 = (T)t1d;
+
+  bool use_thinc = wmin(wmin(ci[0], ci[1]), wmin(ci[2], ci[3])) < THINC_THRESH;
+  bool use_smooth = false;
+  if (SHEAR != 0) use_smooth = wmax(wmax(sg[0], sg[1]), wmax(sg[2], sg[3])) < SIGMA_SMOOTH;
+
+  // face-averaged state, SC:1039-1048
+  const int rA = r0, rB = r0 + s;
+  T ar1a = T(0.5) * ((T)S.q[0][rA] + (T)S.q[0][rB]);
+  T ar2a = T(0.5) * ((T)S.q[1][rA] + (T)S.q[1][rB]);
+  T a1a = T(0.5) * ((T)S.q[5][rA] + (T)S.q[5][rB]);
+  T pa = T(0.5) * ((T)S.q[4][rA] + (T)S.q[4][rB]);
+  T rhoa = ar1a + ar2a;
+  T u_avg = dv(T(0.5) * ((T)S.q[2][rA] + (T)S.q[2][rB]), rhoa);
+  T v_avg = dv(T(0.5) * ((T)S.q[3][rA] + (T)S.q[3][rB]), rhoa);
+  T ca = sound_speed<T>(e, rhoa, pa, a1a);
+  Basis<T> B = make_basis<DIR>(ca, rhoa, ar1a, ar2a, u_avg, v_avg);
+
+  // characteristic projection of the stencil cells -2..3 (SC:1051-1056)
+  T w[6][6];
+#pragma unroll
+  for (int k = klo; k < khi; ++k) {
+    T u[6];
+    const int r = r0 + (k - 2) * s;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) u[m] = (T)S.q[m][r];
+    to_char<DIR>(B, u, w[k]);
+  }
+  T wl[6], wr[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    if (BASE == 0) {
+      wl[m] = muscl_l(w[1][m], w[2][m], w[3][m]);
+      wr[m] = muscl_r(w[2][m], w[3][m], w[4][m]);
+    } else {
+      wl[m] = mp5_l(w[0][m], w[1][m], w[2][m], w[3][m], w[4][m]);
+      wr[m] = mp5_r(w[1][m], w[2][m], w[3][m], w[4][m], w[5][m]);
+    }
+  }
+  if (SHEAR != 0 && use_smooth) {
+    T iph = SHEAR == 1 ? central4(w[1][3], w[2][3], w[3][3], w[4][3])
+                       : central6(w[0][3], w[1][3], w[2][3], w[3][3], w[4][3], w[5][3]);
+    wl[3] = iph;
+    wr[3] = iph;
+  }
+  if (use_thinc) {
+    wl[1] = thinc_l(w[1][1], w[2][1], w[3][1], t1);
+    wl[2] = thinc_l(w[1][2], w[2][2], w[3][2], t1);
+    wr[1] = thinc_r(w[2][1], w[3][1], w[4][1], t1);
+    wr[2] = thinc_r(w[2][2], w[3][2], w[4][2], t1);
+  }
+  if (use_thinc || vf_always) {
+    wl[4] = thinc_l(w[1][4], w[2][4], w[3][4], t1);
+    wr[4] = thinc_r(w[2][4], w[3][4], w[4][4], t1);
+  }
+  T phL[6], phR[6], phfL[6], phfR[6];
+  to_phys<DIR>(B, wl, phL);
+  to_phys<DIR>(B, wr, phR);
+  to_phys<DIR>(B, w[2], phfL);
+  to_phys<DIR>(B, w[3], phfR);
+
+  if (FACEPP == 0) {  // SC:1103-1111
+    if (phL[0] < T(0) || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < T(0) || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < T(0) || phL[5] > T(1)) phL[5] = phfL[5];
+    if (isnan_(phL[4])) phL[4] = phfL[4];
+    if (phR[0] < T(0) || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < T(0) || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < T(0) || phR[5] > T(1)) phR[5] = phfR[5];
+    if (isnan_(phR[4])) phR[4] = phfR[4];
+  } else if constexpr (sizeof(T) == sizeof(double)) {  // M10:1007-1021
+    pp_limit_state(e, phL, phfL);
+    pp_limit_state(e, phR, phfR);
+    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
+    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
+    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
+    if (rho_ct2_sc(e, phL) < PP_CTILDE || isnan_(phL[4])) phL[4] = phfL[4];
+    floor_pressure(e, phL);
+    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
+    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
+    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
+    if (rho_ct2_sc(e, phR) < PP_CTILDE || isnan_(phR[4])) phR[4] = phfR[4];
+    floor_pressure(e, phR);
+  } else {  // the same on FP64 copies: the thresholds are always applied in FP64
+    double dL[6], dR[6], dfL[6], dfR[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      dL[m] = phL[m]; dR[m] = phR[m]; dfL[m] = phfL[m]; dfR[m] = phfR[m];
+    }
+    pp_limit_state(e, dL, dfL);
+    pp_limit_state(e, dR, dfR);
+    if (dL[0] < 0.0 || isnan_(dL[0])) dL[0] = dfL[0];
+    if (dL[1] < 0.0 || isnan_(dL[1])) dL[1] = dfL[1];
+    if (dL[5] < 0.0 || dL[5] > 1.0) dL[5] = dfL[5];
+    if (rho_ct2_sc(e, dL) < PP_CTILDE || isnan_(dL[4])) dL[4] = dfL[4];
+    floor_pressure(e, dL);
+    if (dR[0] < 0.0 || isnan_(dR[0])) dR[0] = dfR[0];
+    if (dR[1] < 0.0 || isnan_(dR[1])) dR[1] = dfR[1];
+    if (dR[5] < 0.0 || dR[5] > 1.0) dR[5] = dfR[5];
+    if (rho_ct2_sc(e, dR) < PP_CTILDE || isnan_(dR[4])) dR[4] = dfR[4];
+    floor_pressure(e, dR);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      phL[m] = (T)dL[m];
+      phR[m] = (T)dR[m];
+    }
+  }
+  if (ca < T(0) || isnan_(ca)) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      phL[m] = phfL[m];
+      phR[m] = phfR[m];
+    }
+  }
+  // primitives for HLLC, SC:1118-1125
+  T rhoL = phL[0] + phL[1], rhoR = phR[0] + phR[1];
+  T PL[6] = {phL[0], phL[1], dv(phL[2], rhoL), dv(phL[3], rhoL), phL[4], phL[5]};
+  T PR[6] = {phR[0], phR[1], dv(phR[2], rhoR), dv(phR[3], rhoR), phR[4], phR[5]};
+  if constexpr (sizeof(T) == sizeof(double)) {
+    hllc<DIR>(e, PL, PR, out, out[6]);
+  } else {
+    hllc_mixed<DIR>(e, PL, PR, out, out[6]);
+  }
+}
+
+// First-order flux (fx_low_kernel SC:1303-1347 / gy_low_kernel SC:1350-1394).
+template <class TL, int DIR, typename T>
+__device__ __forceinline__ void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) {
+  constexpr int s = DIR == 0 ? TL::RJ : 1;
+  const int rL = r0, rR = r0 + s;
+  double rhoL = S.q[0][rL] + S.q[1][rL];
+  double rhoR = S.q[0][rR] + S.q[1][rR];
+  double uL = ddiv(S.q[2][rL], rhoL);
+  double vL = ddiv(S.q[3][rL], rhoL);
+  double uR = ddiv(S.q[2][rR], rhoR);
+  double vR = ddiv(S.q[3][rR], rhoR);
+  double aL = clamp01_pp(S.q[5][rL]);
+  double aR = clamp01_pp(S.q[5][rR]);
+  double pL = S.q[4][rL];
+  double pR = S.q[4][rR];
+  double gL, piL, gR, piR;
+  mix_g_pi(e, aL, gL, piL);
+  mix_g_pi(e, aR, gR, piR);
+  if (isnan_(pL) || ddiv(pL + piL, gL - 1.0) < PP_CTILDE) pL = PP_CTILDE * (gL - 1.0) - piL;
+  if (isnan_(pR) || ddiv(pR + piR, gR - 1.0) < PP_CTILDE) pR = PP_CTILDE * (gR - 1.0) - piR;
+  T PL[6] = {(T)S.q[0][rL], (T)S.q[1][rL], (T)uL, (T)vL, (T)pL, (T)aL};
+  T PR[6] = {(T)S.q[0][rR], (T)S.q[1][rR], (T)uR, (T)vR, (T)pR, (T)aR};
+  if constexpr (sizeof(T) == sizeof(double)) {
+    hllc<DIR>(e, PL, PR, out, out[6]);
+  } else {
+    hllc_mixed<DIR>(e, PL, PR, out, out[6]);
+  }
+}
+
+// One tile of face positions (bi, bj) of one case: stage, indicators, faces.
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const State& U, const Faces& F,
+                                           int vf_always, double t1, int bi, int bj,
+                                           FaceSmem<TL>& S) {
+  constexpr int FTI = TL::TI, FTJ = TL::TJ, FRJ = TL::RJ, FRN = TL::RN;
+  constexpr int FCXN = TL::CXN, FCYN = TL::CYN;
+  const int tid = threadIdx.x;
+  const int i0 = G.g - 1 + bi * FTI;
+  const int j0 = G.g - 1 + bj * FTJ;
+
+  // Phase A: stage the cells of the rectangle, pressure and entropy once.
+  for (int r = tid; r < FRN; r += FTHREADS) {
+    const int i = i0 - 3 + r / FRJ;
+    const int j = j0 - 3 + r % FRJ;
+    double sru, srv;
+    const size_t c = map_cell(G, i, j, sru, srv);
+    double ar1 = U.f[0][c], ar2 = U.f[1][c];
+    double ru = U.f[2][c], rv = U.f[3][c];
+    double E = U.f[4][c], al = U.f[5][c];
+    if (sru < 0.0) ru = -1.0 * ru;
+    if (srv < 0.0) rv = -1.0 * rv;
+    double uu, vv;
+    double p = pressure(e, ar1, ar2, ru, rv, E, al, uu, vv);
+    S.q[0][r] = ar1;
+    S.q[1][r] = ar2;
+    S.q[2][r] = ru;
+    S.q[3][r] = rv;
+    S.q[4][r] = p;
+    S.q[5][r] = al;
+    S.ent[r] = entropy(e, ar1, ar2, p, al);
+  }
+  __syncthreads();
+
+  // Phase B: entropy indicator (cator2_val SC:318-325) and Ducros sensor
+  // once per cell, along each sweep direction.
+  for (int k = tid; k < FCXN; k += FTHREADS) {
+    const int r = (k / FTJ + 2) * FRJ + (k % FTJ + 3);
+    S.cx[k] = cator2(S.ent[r - 2 * FRJ], S.ent[r - FRJ], S.ent[r], S.ent[r + FRJ], S.ent[r + 2 * FRJ]);
+    if (SHEAR != 0) S.sx[k] = ducros_sigma<TL, 0>(S, r);
+  }
+  for (int k = tid; k < FCYN; k += FTHREADS) {
+    const int r = (k / (FTJ + 3) + 3) * FRJ + (k % (FTJ + 3) + 2);
+    S.cy[k] = cator2(S.ent[r - 2], S.ent[r - 1], S.ent[r], S.ent[r + 1], S.ent[r + 2]);
+    if (SHEAR != 0) S.sy[k] = ducros_sigma<TL, 1>(S, r);
+  }
+  __syncthreads();
+
+  // Phase C: one x-face and one y-face per thread.
+  const int ti = tid / FTJ, tj = tid % FTJ;
+  const int i = i0 + ti, j = j0 + tj;
+  const int r0 = (ti + 3) * FRJ + (tj + 3);
+  const size_t o = (size_t)i * (size_t)G.ld + (size_t)j;
+  if (i >= G.g - 1 && i <= G.g + G.nx - 1 && j >= G.g && j <= G.g + G.ny - 1) {
+    double ci[4], sg[4], out[7];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ci[k] = S.cx[(ti + k) * FTJ + tj];
+      sg[k] = SHEAR != 0 ? S.sx[(ti + k) * FTJ + tj] : 0.0;
+    }
+    face_high<TL, 0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F[m][o] = out[m];
+    face_low<TL, 0, T>(e, S, r0, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F[14 + m][o] = out[m];
+  }
+  if (i >= G.g && i <= G.g + G.nx - 1 && j >= G.g - 1 && j <= G.g + G.ny - 1) {
+    double ci[4], sg[4], out[7];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ci[k] = S.cy[ti * (FTJ + 3) + tj + k];
+      sg[k] = SHEAR != 0 ? S.sy[ti * (FTJ + 3) + tj + k] : 0.0;
+    }
+    face_high<TL, 1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F[7 + m][o] = out[m];
+    face_low<TL, 1, T>(e, S, r0, out);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) F[21 + m][o] = out[m];
+  }
+}
+
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(FTHREADS, (BASE == 0 || sizeof(T) == 4) ? 3 : 2)
+k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
+  __shared__ FaceSmem<TL> S;
+  faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, (int)blockIdx.y,
+                                         (int)blockIdx.x, S);
+}
+
+__device__ __forceinline__ int find_case(const int* __restrict__ off, int n, int t) {
+  int lo = 0, hi = n;  // off[lo] <= t < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(FTHREADS, (BASE == 0 || sizeof(T) == 4) ? 3 : 2)
+k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off, int ncase,
+              BatchScal sc, int stage, int vf_always, double t1) {
+  __shared__ FaceSmem<TL> S;
+  const int total = tile_off[ncase];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int k = find_case(tile_off, ncase, t);
+    if (sc.done[k]) continue;  // block-uniform
+    const CaseDesc& c = cs[k];
+    const int lt = t - tile_off[k];
+    State U;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) U.f[m] = c.u[stage][m];
+    const Faces F = c.f;
+    faces_tile<TL, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, F, vf_always, t1, lt / c.tiles_j,
+                                           lt % c.tiles_j, S);
+    __syncthreads();  // the next tile overwrites the LDS
+  }
+}
+
+WAMP_DEV_END

@@ -16,27 +16,51 @@ except ImportError:  # run as a script
 
 REPO = os.path.dirname(os.path.dirname(CSRC_DIR))
 INCLUDE = os.path.join(REPO, "include")
-SOURCES = ["wamp_kernels.hip", "wamp_capi.cpp"]
+OBJ_DIR = os.path.join(CSRC_DIR, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # FP64 parity: keep the reference's expression order, no FMA contraction
-    "-ffp-contract=off",
+COMMON = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed-FP32 instructions in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# (source, contraction): the FP64 kernels and the C-ABI keep the reference's
+# rounding (no FMA contraction, FP64 parity); the FP32-mixed face kernels
+# contract a*b+c within an expression.
+UNITS = [
+    ("wamp_kernels.hip", "-ffp-contract=off"),
+    ("wamp_faces_mixed.hip", "-ffp-contract=on"),
+    ("wamp_capi.cpp", "-ffp-contract=off"),
+]
+
+
+def _run(cmds, verbose):
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                      text=True))
+    failed = False
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
 
 
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC_DIR, s) for s in SOURCES]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs, cmds = [], []
+    for src, contract in UNITS:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *COMMON, contract, "-I", INCLUDE, "-c",
+                     os.path.join(CSRC_DIR, src), "-o", obj])
+    _run(cmds, verbose)  # the units compile in parallel
     tmp = LIB_PATH + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-I", INCLUDE, *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
+    _run([[HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]], verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
