@@ -97,12 +97,17 @@ def test_200_steps_match_oracle(gpu_ok, scheme):
     assert max(errs) <= 1e-9
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("scheme", ["m10", "mp5"])
-def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, monkeypatch):
-    """The register-marching and the LDS-tile limiter/update kernels do the same arithmetic."""
+def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, precision, monkeypatch):
+    """The register-marching and the LDS-tile limiter/update kernels do the same
+    arithmetic, on FP64 face planes and on the mixed storage."""
+    import dataclasses
+
     from wamp.solver import Solver
 
     cfg, orc = _setup(scheme, nx=200, ny=120, presteps=30)
+    cfg = dataclasses.replace(cfg, precision=precision)
     U = orc.state()
     dt = orc.get_dt()
     outs = {}

@@ -7,9 +7,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("lu_kind", ["0", "1"])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("scheme", ["sc", "m10", "mp5"])
-def test_split_calls_equal_one_step(gpu_ok, scheme, precision):
+def test_split_calls_equal_one_step(gpu_ok, scheme, precision, lu_kind, monkeypatch):
+    """In mixed precision wamp_step keeps the face fluxes in the mixed storage
+    (FP32 planes, FP64 energy planes) while the split calls pass 28 FP64 planes:
+    both limiter kernels (lu_kind 0: LDS tile, 1: march) must read the same
+    numbers from either."""
     import dataclasses
 
     import torch
@@ -18,6 +23,7 @@ def test_split_calls_equal_one_step(gpu_ok, scheme, precision):
     from wamp.config import SCHEMES
     from wamp.solver import Solver
 
+    monkeypatch.setenv("WAMP_LU_KIND", lu_kind)
     cfg, U0 = cases.shock_cylinder(160, 80, scheme=SCHEMES[scheme])
     cfg = dataclasses.replace(cfg, precision=precision)
     ref = Solver(cfg)

@@ -54,7 +54,7 @@ hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int 
                                    int blocks, hipStream_t st);
 hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int ncase,
                                 const BatchScal* sc, int stage, int repair, int seg, int blocks,
-                                hipStream_t st);
+                                int f32, hipStream_t st);
 hipError_t wamp_launch_batch_finalize(const CaseDesc* cs, int ncase, const BatchScal* sc,
                                       hipStream_t st);
 hipError_t wamp_launch_batch_dt0(const Geo* G, const Eos* e, const State* U, double* acc,
@@ -119,6 +119,23 @@ extern "C" {
 
 const char* wamp_last_error(void) { return g_err.c_str(); }
 
+// Bytes of the face planes of one case: 28 FP64 planes, or, in mixed
+// precision, the four FP64 energy planes and 24 FP32 planes (the mixed face
+// kernels' other fluxes are FP32 values; see Faces).
+static size_t face_bytes(size_t plane, int precision) {
+  return precision == 0 ? 28 * plane * sizeof(double)
+                        : 4 * plane * sizeof(double) + 24 * plane * sizeof(float);
+}
+// Faces over `bytes = face_bytes(plane, precision)` at p (8-byte aligned).
+static Faces carve_faces(char* p, size_t plane, int precision) {
+  Faces f{(double*)p, (long long)plane};
+  if (precision != 0) {
+    f.h = (float*)(p + 4 * plane * sizeof(double));
+    f.hs = (long long)plane;
+  }
+  return f;
+}
+
 int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_scheme* scheme,
                     int32_t precision, wamp_ctx** out) {
   if (!out) return fail(-1, "out is NULL");
@@ -162,14 +179,13 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   if (rc) { delete c; return rc; }
 
   const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
-  const size_t n_planes = 28 + 12;
-  const size_t bytes = (n_planes * plane + 32) * sizeof(double);
+  const size_t fbytes = face_bytes(plane, precision);  // a multiple of 8
+  const size_t bytes = fbytes + (12 * plane + 32) * sizeof(double);
   rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(scratch)");
   if (rc) { delete c; return rc; }
   c->bytes = (int64_t)bytes;
-  double* p = c->block;
-  c->faces = Faces{p, (long long)plane};
-  p += 28 * plane;
+  c->faces = carve_faces((char*)c->block, plane, precision);
+  double* p = (double*)((char*)c->block + fbytes);
   for (int m = 0; m < 6; ++m, p += plane) c->u1.f[m] = p;
   for (int m = 0; m < 6; ++m, p += plane) c->u2.f[m] = p;
   c->dt_acc = p;
@@ -589,18 +605,20 @@ int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, cons
     b->scratch[slot] = nullptr;
   }
   const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
-  double* p = nullptr;
-  if (int rc = hip_check(hipMalloc((void**)&p, 40 * plane * sizeof(double)), "hipMalloc(case scratch)"))
+  const size_t fbytes = face_bytes(plane, b->precision);
+  char* base = nullptr;
+  if (int rc = hip_check(hipMalloc((void**)&base, fbytes + 12 * plane * sizeof(double)),
+                         "hipMalloc(case scratch)"))
     return rc;
-  b->scratch[slot] = p;
+  b->scratch[slot] = (double*)base;
   CaseDesc& d = b->desc[slot];
   d = CaseDesc();
   d.geo = Geo{grid->nx, grid->ny, grid->g, grid->ld, grid->dx, grid->dy,
               {grid->bc[0], grid->bc[1], grid->bc[2], grid->bc[3]}};
   d.eos = make_eos(*eos);
   for (int m = 0; m < 6; ++m) d.u[0][m] = U[m];
-  d.f = Faces{p, (long long)plane};
-  p += 28 * plane;
+  d.f = carve_faces(base, plane, b->precision);
+  double* p = (double*)(base + fbytes);
   for (int m = 0; m < 6; ++m, p += plane) d.u[1][m] = p;
   for (int m = 0; m < 6; ++m, p += plane) d.u[2][m] = p;
   d.cfl = b->scheme.cfl;
@@ -719,7 +737,8 @@ static int batch_enqueue_step(wamp_batch* b, hipStream_t st) {
                                                    b->precision, b->tile, fb, st), "faces_batch"))
       return rc;
     if (int rc = hip_check(wamp_launch_lu_batch(b->desc_dev, b->off_dev + n + 1, n, &b->sc, s,
-                                                b->scheme.repair, b->lu_seg, lb, st), "lu_batch"))
+                                                b->scheme.repair, b->lu_seg, lb,
+                                                b->precision != 0, st), "lu_batch"))
       return rc;
   }
   return hip_check(wamp_launch_batch_finalize(b->desc_dev, n, &b->sc, st), "finalize");

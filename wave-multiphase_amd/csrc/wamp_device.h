@@ -101,6 +101,32 @@ __device__ __forceinline__ double quotient(double a, double y, double b) {
   return fma(r, y, q);
 }
 
+// Face plane m at cell offset c in either storage of Faces.  The limiter
+// kernels take the storage as a template argument (F32: mixed storage).
+template <bool F32>
+__device__ __forceinline__ double fload(const Faces& F, int m, size_t c) {
+  if constexpr (F32) {
+    if (face_is_energy(m)) return F.b[(long long)(m / 7) * F.s + (long long)c];
+    return (double)F.h[(long long)face_slot32(m) * F.hs + (long long)c];
+  } else {
+    return F.b[(long long)m * F.s + (long long)c];
+  }
+}
+// Store of the face kernels: the FP64 kernels always write FP64 storage; the
+// mixed ones write whichever storage F describes (the split entry point
+// wamp_compute_fluxes hands them 28 FP64 planes).
+template <typename T>
+__device__ __forceinline__ void fstore(const Faces& F, int m, size_t c, double v) {
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (F.h) {
+      if (face_is_energy(m)) F.b[(long long)(m / 7) * F.s + (long long)c] = v;
+      else F.h[(long long)face_slot32(m) * F.hs + (long long)c] = (float)v;
+      return;
+    }
+  }
+  F.b[(long long)m * F.s + (long long)c] = v;
+}
+
 // Reference constants, SC:44-70.
 constexpr double TINY = 1.0e-20;
 constexpr double EPSM = 1.0e-20;

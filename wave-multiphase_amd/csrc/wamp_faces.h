@@ -312,10 +312,10 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
     }
     face_high<TL, 0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F[m][o] = out[m];
+    for (int m = 0; m < 7; ++m) fstore<T>(F, m, o, out[m]);
     face_low<TL, 0, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F[14 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) fstore<T>(F, 14 + m, o, out[m]);
   }
   if (i >= G.g && i <= G.g + G.nx - 1 && j >= G.g - 1 && j <= G.g + G.ny - 1) {
     double ci[4], sg[4], out[7];
@@ -326,10 +326,10 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
     }
     face_high<TL, 1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F[7 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) fstore<T>(F, 7 + m, o, out[m]);
     face_low<TL, 1, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) F[21 + m][o] = out[m];
+    for (int m = 0; m < 7; ++m) fstore<T>(F, 21 + m, o, out[m]);
   }
 }
 

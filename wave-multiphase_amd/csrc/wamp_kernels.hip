@@ -66,25 +66,26 @@ __device__ __forceinline__ double face_theta(double ta, double tb) {
 
 // Residual of component m (0..5) at cell c from blended (rh) and low (rl)
 // face fluxes; m == 5 adds the alpha source with alpha = a (SC:1444-1460).
+template <bool F32>
 __device__ __forceinline__ void residual(const Faces& F, int m, size_t c, size_t ld,
                                          const CellFaces& cf, int nb, double inv_dx,
                                          double inv_dy, double a, double& rh, double& rl) {
-  const double *fx = F[m], *gy = F[7 + m], *fxl = F[14 + m], *gyl = F[21 + m];
   const size_t cx = c - ld, cy = c - 1;
-  double Lxp = fxl[c], Lxm = fxl[cx], Lyp = gyl[c], Lym = gyl[cy];
-  double Hxp = blend(fx[c], Lxp, cf.t[1], nb);
-  double Hxm = blend(fx[cx], Lxm, cf.t[0], nb);
-  double Hyp = blend(gy[c], Lyp, cf.t[3], nb);
-  double Hym = blend(gy[cy], Lym, cf.t[2], nb);
+  double Lxp = fload<F32>(F, 14 + m, c), Lxm = fload<F32>(F, 14 + m, cx);
+  double Lyp = fload<F32>(F, 21 + m, c), Lym = fload<F32>(F, 21 + m, cy);
+  double Hxp = blend(fload<F32>(F, m, c), Lxp, cf.t[1], nb);
+  double Hxm = blend(fload<F32>(F, m, cx), Lxm, cf.t[0], nb);
+  double Hyp = blend(fload<F32>(F, 7 + m, c), Lyp, cf.t[3], nb);
+  double Hym = blend(fload<F32>(F, 7 + m, cy), Lym, cf.t[2], nb);
   rh = (Hxp - Hxm) * inv_dx + (Hyp - Hym) * inv_dy;
   rl = (Lxp - Lxm) * inv_dx + (Lyp - Lym) * inv_dy;
   if (m == 5) {
-    const double *sx = F[6], *sy = F[13], *sxl = F[20], *syl = F[27];
-    double sLxp = sxl[c], sLxm = sxl[cx], sLyp = syl[c], sLym = syl[cy];
-    double sHxp = blend(sx[c], sLxp, cf.t[1], nb);
-    double sHxm = blend(sx[cx], sLxm, cf.t[0], nb);
-    double sHyp = blend(sy[c], sLyp, cf.t[3], nb);
-    double sHym = blend(sy[cy], sLym, cf.t[2], nb);
+    double sLxp = fload<F32>(F, 20, c), sLxm = fload<F32>(F, 20, cx);
+    double sLyp = fload<F32>(F, 27, c), sLym = fload<F32>(F, 27, cy);
+    double sHxp = blend(fload<F32>(F, 6, c), sLxp, cf.t[1], nb);
+    double sHxm = blend(fload<F32>(F, 6, cx), sLxm, cf.t[0], nb);
+    double sHyp = blend(fload<F32>(F, 13, c), sLyp, cf.t[3], nb);
+    double sHym = blend(fload<F32>(F, 13, cy), sLym, cf.t[2], nb);
     rh = rh - a * (sHxp - sHxm) * inv_dx - a * (sHyp - sHym) * inv_dy;
     rl = rl - a * (sLxp - sLxm) * inv_dx - a * (sLyp - sLym) * inv_dy;
   }
@@ -152,6 +153,7 @@ __device__ __forceinline__ void block_min_atomic(double v, double* acc) {
   }
 }
 
+template <bool F32>
 __global__ void __launch_bounds__(LTHREADS)
 k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
                const double* __restrict__ dt_dev, double* dt_acc, int repair) {
@@ -196,7 +198,7 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
 #pragma unroll
           for (int m = 0; m < 2; ++m) {
             double rh, rl;
-            residual(F, m, c, ld, cf, 0, inv_dx, inv_dy, al, rh, rl);
+            residual<F32>(F, m, c, ld, cf, 0, inv_dx, inv_dy, al, rh, rl);
             double u = U.f[m][c];
             l[m] = u - dt * rl;
             h[m] = u - dt * rh;
@@ -204,7 +206,7 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
           theta = theta_stage<0>(e, l, h);
         } else if (s == 1) {
           double rh, rl;
-          residual(F, 5, c, ld, cf, 1, inv_dx, inv_dy, al, rh, rl);
+          residual<F32>(F, 5, c, ld, cf, 1, inv_dx, inv_dy, al, rh, rl);
           l[5] = al - dt * rl;
           h[5] = al - dt * rh;
           theta = theta_stage<1>(e, l, h);
@@ -212,7 +214,7 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
 #pragma unroll
           for (int m = 0; m < 6; ++m) {
             double rh, rl;
-            residual(F, m, c, ld, cf, 2, inv_dx, inv_dy, al, rh, rl);
+            residual<F32>(F, m, c, ld, cf, 2, inv_dx, inv_dy, al, rh, rl);
             double u = U.f[m][c];
             l[m] = u - dt * rl;
             h[m] = u - dt * rh;
@@ -238,7 +240,7 @@ k_limit_update(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha,
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       double rh, rl;
-      residual(F, m, c, ld, cf, 3, inv_dx, inv_dy, ap, rh, rl);
+      residual<F32>(F, m, c, ld, cf, 3, inv_dx, inv_dy, ap, rh, rl);
       un[m] = alpha * Un.f[m][c] + beta * (U.f[m][c] - dt * rh);
     }
     if (repair) repair_cell(e, un);
@@ -301,7 +303,7 @@ __device__ __forceinline__ double blend3(double H, double L, const double* th, i
 // One wave's work item: columns strip*58-3 .. +64, rows seg*SEG .. +SEG.
 // `cw` is this wave's LU_CARRY x 64 doubles of LDS.  Returns this lane's CFL
 // minimum of the updated cells (1e30 if none or if want_dt is false).
-template <int SEG>
+template <int SEG, bool F32>
 __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const State& Un,
                                           const State& U, const Faces& F, const State& Uo,
                                           double alpha, double beta, double dt, int repair,
@@ -322,6 +324,22 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   auto at = [&](int r) { return (size_t)r * ld + (size_t)j; };
   auto lds = [&](int q) -> double& { return cw[q * MW + lane]; };
   // face component m: high F[base + m], low F[base + 14 + m]; base 0 = x, 7 = y
+  // Face value at row r, column j + dj (dj = 0 or -1), zero where !ok.  On the
+  // mixed storage the load is issued on every lane at a clamped in-allocation
+  // address (rows 0..nx+2g-1, columns 0..ld-1; j >= g-3 >= 1) and masked
+  // afterwards: a masked FP32 load would be converted to FP64 inside its
+  // branch, and that conversion waits for the load.
+  const int nrows = G.nx + 2 * G.g;
+  const size_t jc = (size_t)min(j, G.ld - 1);
+  auto face = [&](bool ok, int m, size_t c, int r, int dj) -> double {  // c = at(r) + dj
+    if constexpr (F32) {
+      const size_t cs = (size_t)min(max(r, 0), nrows - 1) * ld + jc + dj;
+      const double v = fload<true>(F, m, cs);
+      return ok ? v : 0.0;
+    } else {
+      return ok ? fload<false>(F, m, c) : 0.0;
+    }
+  };
 
 #pragma unroll
   for (int q = 0; q < LU_CARRY; ++q) lds(q) = 0.0;
@@ -349,20 +367,20 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
         x01[0][q] = x01[1][q];
         x01[1][q] = x01[2][q];
       }
-      x01[2][0] = ok ? F[0][c] : 0.0;
-      x01[2][1] = ok ? F[1][c] : 0.0;
-      x01[2][2] = ok ? F[14][c] : 0.0;
-      x01[2][3] = ok ? F[15][c] : 0.0;
+      x01[2][0] = face(ok, 0, c, f, 0);
+      x01[2][1] = face(ok, 1, c, f, 0);
+      x01[2][2] = face(ok, 14, c, f, 0);
+      x01[2][3] = face(ok, 15, c, f, 0);
     }
     {
       const bool ok = xrow_in(f - 1) && col_in;
       const size_t c = at(f - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) x56[0][q] = x56[1][q];
-      x56[1][0] = ok ? F[5][c] : 0.0;
-      x56[1][1] = ok ? F[6][c] : 0.0;
-      x56[1][2] = ok ? F[19][c] : 0.0;
-      x56[1][3] = ok ? F[20][c] : 0.0;
+      x56[1][0] = face(ok, 5, c, f - 1, 0);
+      x56[1][1] = face(ok, 6, c, f - 1, 0);
+      x56[1][2] = face(ok, 19, c, f - 1, 0);
+      x56[1][3] = face(ok, 20, c, f - 1, 0);
     }
     {
       // row f-2: components 0, 1 and 5, 6 are already held (same predicate)
@@ -380,8 +398,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       xa[1][13] = x56[0][3];
 #pragma unroll
       for (int m = 2; m < 5; ++m) {
-        xa[1][m] = ok ? F[m][c] : 0.0;
-        xa[1][7 + m] = ok ? F[14 + m][c] : 0.0;
+        xa[1][m] = face(ok, m, c, f - 2, 0);
+        xa[1][7 + m] = face(ok, 14 + m, c, f - 2, 0);
       }
     }
 
@@ -393,15 +411,15 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const size_t c = at(f);
       const bool ok = row_in(f) && col_in;
       const bool yok = row_in(f) && yface_in;
-      yc[0] = yok ? F[7][c] : 0.0;
-      yc[1] = yok ? F[8][c] : 0.0;
-      yc[2] = yok ? F[21][c] : 0.0;
-      yc[3] = yok ? F[22][c] : 0.0;
+      yc[0] = face(yok, 7, c, f, 0);
+      yc[1] = face(yok, 8, c, f, 0);
+      yc[2] = face(yok, 21, c, f, 0);
+      yc[3] = face(yok, 22, c, f, 0);
       // lower y-faces are loaded (not shuffled) so theta0 is exact on every lane
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
-        yd0[m] = ok ? F[7 + m][c - 1] : 0.0;
-        yd0[2 + m] = ok ? F[21 + m][c - 1] : 0.0;
+        yd0[m] = face(ok, 7 + m, c - 1, f, -1);
+        yd0[2 + m] = face(ok, 21 + m, c - 1, f, -1);
       }
       u01[0] = ok ? U.f[0][c] : 0.0;
       u01[1] = ok ? U.f[1][c] : 0.0;
@@ -411,10 +429,10 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const size_t c = at(f - 1);
       const bool ok = row_in(f - 1) && col_in;
       const bool yok = row_in(f - 1) && yface_in;
-      yu[0] = yok ? F[12][c] : 0.0;
-      yu[1] = yok ? F[13][c] : 0.0;
-      yu[2] = yok ? F[26][c] : 0.0;
-      yu[3] = yok ? F[27][c] : 0.0;
+      yu[0] = face(yok, 12, c, f - 1, 0);
+      yu[1] = face(yok, 13, c, f - 1, 0);
+      yu[2] = face(yok, 26, c, f - 1, 0);
+      yu[3] = face(yok, 27, c, f - 1, 0);
       al1 = ok ? U.f[5][c] : 0.0;
     }
     double yn24[6], u234[3];  // theta2, row f-2: y-face components 2..4 (H, L), U2..U4
@@ -424,8 +442,8 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       const bool yok = row_in(f - 2) && yface_in;
 #pragma unroll
       for (int m = 2; m < 5; ++m) {
-        yn24[m - 2] = yok ? F[7 + m][c] : 0.0;
-        yn24[m + 1] = yok ? F[21 + m][c] : 0.0;
+        yn24[m - 2] = face(yok, 7 + m, c, f - 2, 0);
+        yn24[m + 1] = face(yok, 21 + m, c, f - 2, 0);
       }
       u234[0] = ok ? U.f[2][c] : 0.0;
       u234[1] = ok ? U.f[3][c] : 0.0;
@@ -638,7 +656,7 @@ __device__ __forceinline__ void wave_min_atomic(double v, double* acc) {
     atomicMin((unsigned long long*)acc, (unsigned long long)__double_as_longlong(v));
 }
 
-template <int SEG>
+template <int SEG, bool F32>
 __global__ void __launch_bounds__(256, 2)
 k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
            const double* __restrict__ dt_dev, double* dt_acc, int repair) {
@@ -646,7 +664,7 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
   const int seg = (int)blockIdx.y * (blockDim.x / MW) + wave;
   if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
-  double v = lu_wave<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
+  double v = lu_wave<SEG, F32>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
                           (int)blockIdx.x, seg, carry + wave * (LU_CARRY * MW));
   if (dt_acc) wave_min_atomic(v, dt_acc);
 }
@@ -777,7 +795,7 @@ __global__ void __launch_bounds__(STHREADS) k_bc(Geo G, State U, double* up, dou
 // ========================================================================
 
 // One wave per work item (case, strip, segment) of the concatenated list.
-template <int SEG>
+template <int SEG, bool F32>
 __global__ void __launch_bounds__(256, 2)
 k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, int ncase,
            BatchScal sc, int stage, int repair) {
@@ -800,8 +818,9 @@ k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, in
     U.f[m] = c.u[stage][m];
     Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
   }
-  double v = lu_wave<SEG>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair, stage == 2,
-                          li % c.strips, li / c.strips, carry + wave * (LU_CARRY * MW));
+  double v = lu_wave<SEG, F32>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair,
+                               stage == 2, li % c.strips, li / c.strips,
+                               carry + wave * (LU_CARRY * MW));
   if (stage == 2) wave_min_atomic(v, sc.acc + k);
 }
 
@@ -966,8 +985,12 @@ hipError_t wamp_launch_limit_update(const Geo* G, const Eos* e, const State* Un,
                                     const double* dt_dev, double* dt_acc, int repair,
                                     hipStream_t st) {
   dim3 grid((G->ny + LT - 1) / LT, (G->nx + LT - 1) / LT);
-  hipLaunchKernelGGL(k_limit_update, grid, dim3(LTHREADS), 0, st, *G, *e, *Un, *U, *F, *Uo, alpha,
-                     beta, dt_dev, dt_acc, repair);
+  if (F->h)
+    hipLaunchKernelGGL(k_limit_update<true>, grid, dim3(LTHREADS), 0, st, *G, *e, *Un, *U, *F, *Uo,
+                       alpha, beta, dt_dev, dt_acc, repair);
+  else
+    hipLaunchKernelGGL(k_limit_update<false>, grid, dim3(LTHREADS), 0, st, *G, *e, *Un, *U, *F,
+                       *Uo, alpha, beta, dt_dev, dt_acc, repair);
   return hipGetLastError();
 }
 
@@ -978,11 +1001,15 @@ hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, con
   const int waves_per_block = 4;
   const int nseg = (G->nx + seg - 1) / seg;
   dim3 grid((G->ny + MJ - 1) / MJ, (nseg + waves_per_block - 1) / waves_per_block);
-#define WAMP_SEG(S)                                                                          \
-  if (seg == S) {                                                                            \
-    hipLaunchKernelGGL(k_lu_march<S>, grid, dim3(64 * waves_per_block), 0, st, *G, *e, *Un, \
-                       *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);                    \
-    return hipGetLastError();                                                                \
+#define WAMP_SEG(S)                                                                             \
+  if (seg == S) {                                                                               \
+    if (F->h)                                                                                   \
+      hipLaunchKernelGGL((k_lu_march<S, true>), grid, dim3(64 * waves_per_block), 0, st, *G, *e, \
+                         *Un, *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);                \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_lu_march<S, false>), grid, dim3(64 * waves_per_block), 0, st, *G,   \
+                         *e, *Un, *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);            \
+    return hipGetLastError();                                                                   \
   }
   WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(32) WAMP_SEG(64)
 #undef WAMP_SEG
@@ -1080,12 +1107,16 @@ hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int 
 
 hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int ncase,
                                 const BatchScal* sc, int stage, int repair, int seg, int blocks,
-                                hipStream_t st) {
-#define WAMP_LB(S)                                                                            \
-  if (seg == S) {                                                                             \
-    hipLaunchKernelGGL(k_lu_batch<S>, dim3(blocks), dim3(256), 0, st, cs, item_off, ncase, *sc, \
-                       stage, repair);                                                        \
-    return hipGetLastError();                                                                 \
+                                int f32, hipStream_t st) {
+#define WAMP_LB(S)                                                                              \
+  if (seg == S) {                                                                               \
+    if (f32)                                                                                    \
+      hipLaunchKernelGGL((k_lu_batch<S, true>), dim3(blocks), dim3(256), 0, st, cs, item_off,    \
+                         ncase, *sc, stage, repair);                                            \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_lu_batch<S, false>), dim3(blocks), dim3(256), 0, st, cs, item_off,   \
+                         ncase, *sc, stage, repair);                                            \
+    return hipGetLastError();                                                                   \
   }
   WAMP_LB(8) WAMP_LB(16) WAMP_LB(32)
 #undef WAMP_LB

@@ -23,12 +23,23 @@ struct State {  // the six conserved planes [a1 rho1, a2 rho2, rho u, rho v, E, 
   double* f[6];
 };
 // The 28 face planes fx0..5, srcx, gy0..5, srcy, fxl0..5, srcxl, gyl0..5,
-// srcyl, equally spaced in one allocation: plane m starts at b + m * s.
+// srcyl (groups of 7: high x, high y, low x, low y).
+//  * FP64 storage (h == nullptr): equally spaced in one allocation, plane m
+//    starts at b + m * s.
+//  * Mixed storage (h != nullptr; the FP32-mixed face kernels, whose fluxes
+//    other than the energy flux are FP32 values): the four energy planes
+//    (m % 7 == 4) in FP64 at b + (m / 7) * s, the other 24 in FP32 at
+//    h + face_slot32(m) * hs.  Storing those FP32 values in FP32 loses
+//    nothing, so both storages give the limiter the same numbers.
 struct Faces {
   double* b;
   long long s;
-  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }
+  float* h = nullptr;
+  long long hs = 0;
+  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }  // FP64 storage
 };
+__host__ __device__ constexpr int face_slot32(int m) { return m - m / 7 - (m % 7 > 4 ? 1 : 0); }
+__host__ __device__ constexpr bool face_is_energy(int m) { return m % 7 == 4; }
 
 // One case of a batch (device-resident table).
 struct CaseDesc {

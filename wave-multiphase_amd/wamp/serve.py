@@ -28,7 +28,9 @@ from . import cases
 from .batch import Batch, Tuning
 from .config import PREC_MIXED, SCHEME_MP5, CaseConfig
 
-PLANES_PER_CASE = 46  # 6 state planes + 28 face planes + 2 x 6 stage planes
+# device memory of a case in FP64 planes: 6 state + 2 x 6 stage planes + the 28
+# face planes, which mixed precision keeps as 4 FP64 (energy) + 24 FP32 planes
+PLANES_PER_CASE = {0: 6 + 12 + 28, 1: 6 + 12 + 4 + 12}
 
 
 @dataclass(order=True)
@@ -49,7 +51,7 @@ class Request:
     @property
     def nbytes(self) -> int:
         g = self.cfg.grid
-        return PLANES_PER_CASE * g.nxg * g.nyg * 8
+        return PLANES_PER_CASE[self.cfg.precision] * g.nxg * g.nyg * 8
 
 
 @dataclass
