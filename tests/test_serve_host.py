@@ -101,9 +101,16 @@ def _run(reqs, need, max_slots, budget, round_steps=2):
 
 
 def test_request_bytes_follow_the_padded_grid():
+    """State, two stage buffers and the face planes: 28 FP64 planes, or in mixed
+    precision 4 FP64 energy planes + 24 FP32 planes."""
+    import dataclasses
+
     r = _req(0, 0.0, 100, 60)
     g = r.cfg.grid
-    assert r.nbytes == 46 * g.nxg * g.nyg * 8
+    planes = {0: 6 + 12 + 28, 1: 6 + 12 + 4 + 24 / 2}
+    for prec, n in planes.items():
+        rp = dataclasses.replace(r, cfg=dataclasses.replace(r.cfg, precision=prec))
+        assert rp.nbytes == n * g.nxg * g.nyg * 8
 
 
 def test_all_requests_finish_with_their_own_state_and_end_time():

@@ -60,11 +60,45 @@ template <> __device__ __forceinline__ float msign<float>(float x) { return __bu
 // v_sqrt_f32 and an exp-based tanh, about 1-2 ulp): on CDNA4 non-packed
 // FP32 issues at the FP64 rate, so the mixed mode gains only by dropping the
 // multi-instruction IEEE sequences.
-// FP64 division.  The FP64 kernels keep the IEEE correctly rounded divide.
-// The mixed unit's remaining FP64 arithmetic (pressure recovery, indicators,
-// face positivity limiter, energy flux) takes the hardware reciprocal, one
-// Newton step and one residual correction (Markstein): within one ulp of the
-// quotient, six instructions instead of the eleven of the IEEE sequence.
+// a / b from y = RN(1/b) with one correction step (Markstein 1990; Muller et
+// al., Handbook of Floating-Point Arithmetic, the FMA-based division
+// theorem): q = RN(a y), r = a - b q (exact with an FMA), RN(q + r y) is the
+// correctly rounded quotient for finite a, b with no overflow or underflow,
+// i.e. bitwise a / b.  Three instructions where y is known (a host constant
+// or a reciprocal shared by several numerators).
+__device__ __forceinline__ double quotient(double a, double y, double b) {
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
+}
+
+// 1/b from v_rcp_f64 and two Newton steps: the reciprocal steps of the
+// compiler's IEEE divide sequence (v_div_scale, v_rcp, 4 x v_fma, v_mul,
+// v_fma, v_div_fmas, v_div_fixup) without its operand scaling.
+__device__ __forceinline__ double recip_nr(double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  return fma(y, e, y);
+}
+
+// FP64 division.
+//  * FP64 kernels: quotient(a, recip_nr(b), b) is the IEEE divide sequence
+//    minus v_div_scale / v_div_fmas / v_div_fixup, which change nothing when
+//    no operand needs scaling or special-casing, so it is a / b bit for bit
+//    for finite b with 2^-1000 <= |b| <= 2^1000 and finite a with a = 0 or
+//    |a| >= 2^-960 and |a / b| in [2^-1000, 2^700] (a zero quotient may come
+//    back with the other sign).  For b = 0 or a non-finite b it gives NaN
+//    where the divide gives +-inf, so it is used where the divisor cannot
+//    vanish in an admissible state (densities, sound speeds, mixture gammas,
+//    limiter denominators kept above TINY); the HLLC ratios over SL - SP and
+//    SR - SP keep the IEEE divide.  Eight instructions instead of eleven,
+//    and the compiler keeps one reciprocal per divisor, so each further
+//    numerator of a shared divisor costs three.
+//  * The mixed unit's remaining FP64 arithmetic (pressure recovery,
+//    indicators, face positivity limiter, energy flux): one Newton step and
+//    the correction, within one ulp of the quotient, six instructions.
 __device__ __forceinline__ double ddiv(double a, double b) {
 #ifdef WAMP_MIXED_TU
   double y = __builtin_amdgcn_rcp(b);
@@ -74,7 +108,7 @@ __device__ __forceinline__ double ddiv(double a, double b) {
   const double r = fma(-b, q, a);
   return fma(r, y, q);
 #else
-  return a / b;
+  return quotient(a, recip_nr(b), b);
 #endif
 }
 __device__ __forceinline__ double dv(double a, double b) { return ddiv(a, b); }
@@ -88,18 +122,6 @@ __device__ __forceinline__ float th(float x) {
   return x < 0.0f ? -r : r;
 }
 
-// a / b from y = RN(1/b) with one correction step (Markstein 1990; Muller et
-// al., Handbook of Floating-Point Arithmetic, the FMA-based division
-// theorem): q = RN(a y), r = a - b q (exact with an FMA), RN(q + r y) is the
-// correctly rounded quotient for finite a, b with no overflow or underflow,
-// i.e. bitwise a / b.  Three instructions instead of the IEEE divide
-// sequence when y is shared (a common denominator or a host constant).
-// Callers use it only where b is finite and non-zero.
-__device__ __forceinline__ double quotient(double a, double y, double b) {
-  const double q = a * y;
-  const double r = fma(-b, q, a);
-  return fma(r, y, q);
-}
 
 // Face plane m at cell offset c in either storage of Faces.  The limiter
 // kernels take the storage as a template argument (F32: mixed storage).
@@ -202,8 +224,8 @@ __device__ __forceinline__ double pressure(const Eos& e, double ar1, double ar2,
                                            double rv, double E, double a1, double& uu,
                                            double& vv) {
   double rho = ar1 + ar2;
-  uu = ddiv(ru, rho);
-  vv = ddiv(rv, rho);
+  uu = dv(ru, rho);
+  vv = dv(rv, rho);
   double gb, pb;
   mix_gb_pb(e, a1, gb, pb);
   return ddiv(E - pb - 0.5 * rho * (uu * uu + vv * vv), gb);
@@ -217,7 +239,7 @@ __device__ __forceinline__ double theta_lower(double base, double high, double f
     theta = 0.0;
   } else if (high < floor_) {
     double denom = base - high;
-    theta = denom > TINY ? (base - floor_) / denom : 0.0;
+    theta = denom > TINY ? ddiv(base - floor_, denom) : 0.0;
   }
   if (theta < 0.0) theta = 0.0;
   if (theta > 1.0) theta = 1.0;
@@ -230,9 +252,8 @@ __device__ __forceinline__ double rho_ct2_cons(const Eos& e, double m1, double m
   if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
   if (m2 < PP_RHO || isnan_(m2)) m2 = PP_RHO;
   double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
-  const double y = 1.0 / rho;
-  double uu = quotient(mx, y, rho);
-  double vv = quotient(my, y, rho);
+  double uu = dv(mx, rho);
+  double vv = dv(my, rho);
   double kin = 0.5 * rho * (uu * uu + vv * vv);
   double g, pi;
   mix_g_pi(e, a1, g, pi);
@@ -391,13 +412,14 @@ __device__ __forceinline__ Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg
   Basis<T> b;
   b.c = c;
   b.inv2c = dv(T(0.5), c);
-  b.inv2c2 = dv(T(0.5), c * c);
+  const T c2 = c * c;
+  b.inv2c2 = dv(T(0.5), c2);
   T Y1 = dv(ar1, rho);
   T Y2 = dv(ar2, rho);
   b.Y1 = Y1;
   b.Y2 = Y2;
-  b.y1c2 = dv(Y1, c * c);
-  b.y2c2 = dv(Y2, c * c);
+  b.y1c2 = dv(Y1, c2);
+  b.y2c2 = dv(Y2, c2);
   T un = DIR == 0 ? u_avg : v_avg;
   b.k = un * b.inv2c;
   b.ut = DIR == 0 ? v_avg : -u_avg;  // to_char_x: ut = v;  to_char_y: ut_f = -u
@@ -513,7 +535,7 @@ __device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f,
   T SR = wmax(nR + cR, n_avg + c_avg);
   T denom = rhoL * (SL - nL) - rhoR * (SR - nR);
   if (fabs(denom) < T(TINY)) denom = T(TINY);
-  T SP = (pR - pL + rhoL * nL * (SL - nL) - rhoR * nR * (SR - nR)) / denom;
+  T SP = dv(pR - pL + rhoL * nL * (SL - nL) - rhoR * nR * (SR - nR), denom);  // |denom| >= TINY
   T ratL = (SL - nL) / (SL - SP);
   T ratR = (SR - nR) / (SR - SP);
 
@@ -528,7 +550,7 @@ __device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f,
     f[5] = a1L * nL;
   } else if (SL <= T(0) && T(0) < SP) {
     T ratio = ratL;
-    T EL_star = EL + (SP - nL) * (SP * rhoL + pL / (SL - nL));
+    T EL_star = EL + (SP - nL) * (SP * rhoL + dv(pL, SL - nL));  // SL - nL <= -cL < 0
     f[0] = ar1L * nL + SL * (ar1L * ratio - ar1L);
     f[1] = ar2L * nL + SL * (ar2L * ratio - ar2L);
     if (DIR == 0) {
@@ -542,7 +564,7 @@ __device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f,
     f[5] = a1L * nL + SL * (a1L * ratio - a1L);
   } else if (SP <= T(0) && T(0) <= SR) {
     T ratio = ratR;
-    T ER_star = ER + (SP - nR) * (SP * rhoR + pR / (SR - nR));
+    T ER_star = ER + (SP - nR) * (SP * rhoR + dv(pR, SR - nR));  // SR - nR >= cR > 0
     f[0] = ar1R * nR + SR * (ar1R * ratio - ar1R);
     f[1] = ar2R * nR + SR * (ar2R * ratio - ar2R);
     if (DIR == 0) {

@@ -67,7 +67,7 @@ struct FaceSmem {
 // Ducros x pressure sensor (ducros_kernel SC:875-916) at LDS cell r.
 template <class TL>
 __device__ __forceinline__ double lds_vel(const FaceSmem<TL>& S, int r, int comp) {
-  return ddiv(S.q[comp][r], S.q[0][r] + S.q[1][r]);
+  return dv(S.q[comp][r], S.q[0][r] + S.q[1][r]);
 }
 template <class TL, int DIR>
 __device__ __forceinline__ double ducros_sigma(const FaceSmem<TL>& S, int r) {
@@ -228,10 +228,10 @@ __device__ __forceinline__ void face_low(const Eos& e, const FaceSmem<TL>& S, in
   const int rL = r0, rR = r0 + s;
   double rhoL = S.q[0][rL] + S.q[1][rL];
   double rhoR = S.q[0][rR] + S.q[1][rR];
-  double uL = ddiv(S.q[2][rL], rhoL);
-  double vL = ddiv(S.q[3][rL], rhoL);
-  double uR = ddiv(S.q[2][rR], rhoR);
-  double vR = ddiv(S.q[3][rR], rhoR);
+  double uL = dv(S.q[2][rL], rhoL);
+  double vL = dv(S.q[3][rL], rhoL);
+  double uR = dv(S.q[2][rR], rhoR);
+  double vR = dv(S.q[3][rR], rhoR);
   double aL = clamp01_pp(S.q[5][rL]);
   double aR = clamp01_pp(S.q[5][rR]);
   double pL = S.q[4][rL];

@@ -100,13 +100,12 @@ __device__ __forceinline__ void repair_cell(const Eos& e, double* u) {
   if (isnan_(mx)) mx = 0.0;
   if (isnan_(my)) my = 0.0;
   double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
-  const double y = 1.0 / rho;
-  double uu = quotient(mx, y, rho);
-  double vv = quotient(my, y, rho);
+  double uu = ddiv(mx, rho);
+  double vv = ddiv(my, rho);
   double kin = 0.5 * rho * (uu * uu + vv * vv);
   double gb, pb;
   mix_gb_pb(e, a1, gb, pb);
-  double p = (E - pb - kin) / gb;
+  double p = ddiv(E - pb - kin, gb);
   double g, pi;
   mix_g_pi(e, a1, g, pi);
   double p_min = PP_CTILDE * (g - 1.0) - pi;
@@ -132,9 +131,9 @@ __device__ __forceinline__ double cell_dt(const Eos& e, const double* u, double 
   double p = pressure(e, u[0], u[1], u[2], u[3], u[4], u[5], uu, vv);
   double rho = u[0] + u[1];
   double c = sound_speed(e, rho, p, u[5]);
-  double sx = dx / (fabs(uu) + c);
-  double sy = dy / (fabs(vv) + c);
-  return sx * sy / (sx + sy);
+  double sx = ddiv(dx, fabs(uu) + c);
+  double sy = ddiv(dy, fabs(vv) + c);
+  return ddiv(sx * sy, sx + sy);
 }
 
 // Block-wide min of a positive double, one atomic per block (bit pattern
