@@ -113,7 +113,29 @@ __device__ __forceinline__ double ddiv(double a, double b) {
 }
 __device__ __forceinline__ double dv(double a, double b) { return ddiv(a, b); }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+// FP64 square root of the sound speed (argument clamped to >= TINY = 1e-20).
+// FP64 kernels: the compiler's IEEE sqrt sequence (v_rsq_f64, two products,
+// seven FMAs) without its scaling of arguments below 2^-767 and its select
+// for zero and +inf, which act only outside this argument range: bitwise
+// sqrt(x) for finite x >= 2^-767; NaN stays NaN (+inf would give NaN, not
+// +inf, but the argument is finite in any admissible state).  The mixed unit keeps the
+// library sqrt for its few FP64 square roots.
+__device__ __forceinline__ double sq(double x) {
+#ifdef WAMP_MIXED_TU
+  return sqrt(x);
+#else
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g;
+  double h = g * 0.5;
+  const double r = fma(-h, s, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  d = fma(-s, s, x);
+  return fma(d, h, s);
+#endif
+}
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double th(double x) { return tanh(x); }
 __device__ __forceinline__ float th(float x) {
