@@ -39,7 +39,8 @@ def test_batch_equals_alone_under_all_tunings(gpu_ok, precision):
     cs = _cases(precision)
     steps = 6
     ref = [_alone(c, U, steps) for c, U in cs]
-    for tuning, graph in ((Tuning(0, 3, 2, 16), 0), (Tuning(1, 2, 4, 8), 3), (Tuning(0, 1, 1, 32), 6)):
+    for tuning, graph in ((Tuning(0, 3, 2, 16), 0), (Tuning(1, 2, 4, 8), 3), (Tuning(0, 1, 1, 32), 6),
+                          (Tuning(1, 3, 2, 26), 2), (Tuning(0, 3, 2, 20), 0)):
         b = Batch(cs[0][0].scheme, precision, max_cases=6)
         for k, (c, U) in enumerate(cs):
             b.set_case(k + 1, c, U)  # slot 0 and 5 stay empty

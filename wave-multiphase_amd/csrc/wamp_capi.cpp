@@ -70,6 +70,7 @@ struct wamp_ctx {
   double t1;          // tanh(beta_thinc / 2), SC:334
   int lu_kind;        // 0: LDS-tile limiter/update kernel, 1: marching kernel, -1: by grid size
   int lu_seg;         // rows per wave of the marching kernel (0: by grid size)
+  int cu_count;       // compute units of the device (limiter segment choice)
   double* block;      // one allocation: faces | stage buffers | dt accumulator
   int64_t bytes;
   Faces faces;
@@ -176,6 +177,9 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   if (const char* v = std::getenv("WAMP_LU_KIND")) c->lu_kind = std::atoi(v);
   if (const char* v = std::getenv("WAMP_LU_SEG")) c->lu_seg = std::atoi(v);
   int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
+  if (!rc)
+    rc = hip_check(hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount,
+                                         c->device), "hipDeviceGetAttribute");
   if (rc) { delete c; return rc; }
 
   const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
@@ -274,15 +278,31 @@ static int faces_of(const wamp_ctx* c, double* const F[28], Faces& f) {
   return 0;
 }
 
-// Rows per wave of the marching limiter.  Each wave's 3-row warm-up and
-// drain make a segment's latency grow with its length; 16 rows is fastest
-// when the grid yields enough waves to occupy every SIMD twice (measured at
-// 4096x2048), shorter segments halve the serial march of small grids.
+// Rows per wave of the marching limiter.  A wave of `seg` rows marches
+// seg + 3 fronts (the 3-row warm-up re-reads rows of the segment below), and
+// the kernel's blocks (4 waves of one strip) run in rounds of 2 blocks per
+// CU.  Cost model: rounds x (seg + 3), over the instantiated lengths; it
+// picks the measured best at 1024x512 (8 rows: 0.098 ms per launch; 16:
+// 0.111), 2048x1024 (20: 0.231; 16: 0.311) and 4096x2048 (26: 0.861; 16:
+// 0.916).
+constexpr int MJ_COLS = 58;  // output columns per limiter wave (64 lanes - 2 x 3 halo)
 static int lu_rows(const wamp_ctx* c) {
   if (c->lu_seg) return c->lu_seg;
-  const long long strips = (c->geo.ny + 57) / 58;
-  const long long waves16 = strips * ((c->geo.nx + 15) / 16);
-  return waves16 < 2048 ? 8 : 16;  // (4 rows: 0.18 ms/step at 128x128, 8: 0.25, 16: 0.39)
+  static const int cand[] = {4, 8, 16, 20, 26, 28, 32};
+  const long long strips = (c->geo.ny + MJ_COLS - 1) / MJ_COLS;
+  const long long slots = 2LL * std::max(1, c->cu_count);
+  int best = 16;
+  long long best_cost = -1;
+  for (int seg : cand) {
+    const long long segs = (c->geo.nx + seg - 1) / seg;
+    const long long blocks = strips * ((segs + 3) / 4);
+    const long long cost = ((blocks + slots - 1) / slots) * (seg + 3);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = seg;
+    }
+  }
+  return best;
 }
 
 // Limiter kernel: the LDS-tile kernel (every tile in parallel, four short
@@ -646,7 +666,8 @@ int wamp_batch_set_tuning(wamp_batch* b, int32_t tile, int32_t faces_bpcu, int32
   if (tile < 0 || tile > 1) return fail(-3, "tile must be 0 (8x32) or 1 (16x16)");
   if (faces_bpcu < 1 || faces_bpcu > 16 || lu_bpcu < 1 || lu_bpcu > 16)
     return fail(-3, "blocks per CU must be in [1, 16]");
-  if (lu_seg != 8 && lu_seg != 16 && lu_seg != 32) return fail(-3, "lu_seg must be 8, 16 or 32");
+  if (lu_seg != 8 && lu_seg != 16 && lu_seg != 20 && lu_seg != 26 && lu_seg != 32)
+    return fail(-3, "lu_seg must be 8, 16, 20, 26 or 32");
   b->tile = tile;
   b->faces_bpcu = faces_bpcu;
   b->lu_bpcu = lu_bpcu;

@@ -1010,7 +1010,7 @@ hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, con
                          *e, *Un, *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);            \
     return hipGetLastError();                                                                   \
   }
-  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(32) WAMP_SEG(64)
+  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(20) WAMP_SEG(26) WAMP_SEG(28) WAMP_SEG(32) WAMP_SEG(64)
 #undef WAMP_SEG
   return hipErrorInvalidValue;
 }
@@ -1117,7 +1117,7 @@ hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int nca
                          ncase, *sc, stage, repair);                                            \
     return hipGetLastError();                                                                   \
   }
-  WAMP_LB(8) WAMP_LB(16) WAMP_LB(32)
+  WAMP_LB(8) WAMP_LB(16) WAMP_LB(20) WAMP_LB(26) WAMP_LB(32)
 #undef WAMP_LB
   return hipErrorInvalidValue;
 }

@@ -272,7 +272,7 @@ def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int 
                   log=None):
     """Run `reqs` (arrivals as offsets from the start) through a Server."""
     srv = Server(scheme=scheme, precision=precision, max_slots=max_slots, hbm_budget=hbm_budget,
-                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 16))
+                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 26))
     base = srv.clock()
     for r in reqs:
         srv.submit(dataclasses.replace(r, arrival=base + r.arrival))
