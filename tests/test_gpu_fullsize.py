@@ -84,7 +84,11 @@ def test_full_size_many_steps_match_oracle(gpu_ok, case):
     orc.set_state(U0)
     orc.prepare()
     U = orc.state()
-    dts_ref = orc.run(n)
+    chunks = []
+    for k0 in range(0, n, 25):  # the same run as orc.run(n), with progress lines
+        chunks.append(orc.run(min(25, n - k0)))
+        print(f"oracle: {k0 + len(chunks[-1])} of {n} steps", flush=True)
+    dts_ref = np.concatenate(chunks)
     ref = orc.state()
     del orc
     s = Solver(cfg)
