@@ -318,7 +318,15 @@ __device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, 
   if (rho < TINY) rho = TINY;
   double g, pi;
   mix_g_pi(e, a1, g, pi);
+#ifdef WAMP_MIXED_TU
+  // mixed unit: rho^g = exp(g log rho) for rho >= TINY.  Its relative error
+  // is about |g log rho| ulps (< 1e-14 for the densities and gammas here)
+  // against the library pow, which also handles negative, zero and infinite
+  // bases; the indicator built on it only gates the reconstruction.
+  return ddiv(p, exp(g * log(rho)));
+#else
   return ddiv(p, pow(rho, g));
+#endif
 }
 // cator2_val SC:318-325
 __device__ __forceinline__ double cator2(double em2, double em1, double e0, double ep1, double ep2) {
@@ -408,18 +416,29 @@ template <typename T>
 __device__ __forceinline__ T central4(T vm1, T v0, T vp1, T vp2) {
   return (-T(1.0 / 12.0) * vm1 + T(7.0 / 12.0) * v0 + T(7.0 / 12.0) * vp1 - T(1.0 / 12.0) * vp2);
 }
-// muscl_char_ul / muscl_char_ur SC:430-446
+// muscl_char_ul / muscl_char_ur SC:430-446:
+// v +- 1/4 ((1 - KAP) mm2(d, 2 e) + (1 + KAP) mm2(e, 2 d)).  Both minmods share
+// their sign factor: sign(2 e) = sign(e) and sign(2 d) = sign(d) (also for
+// zeros), so 0.5 (sign(d) + sign(e)) is formed once; every value is the one
+// mm2 gives.
+template <typename T>
+__device__ __forceinline__ T muscl_pair(T d, T e) {
+  const T s = T(0.5) * (msign(d) + msign(e));
+  const T a = s * wmin(fabs(d), fabs(T(2) * e));  // mm2(d, 2 e)
+  const T b = s * wmin(fabs(e), fabs(T(2) * d));  // mm2(e, 2 d)
+  return T(0.25) * ((T(1) - T(KAP)) * a + (T(1) + T(KAP)) * b);
+}
 template <typename T>
 __device__ __forceinline__ T muscl_l(T vm1, T v0, T vp1) {
   T dm = v0 - vm1;
   T d0 = vp1 - v0;
-  return v0 + T(0.25) * ((T(1) - T(KAP)) * mm2(dm, T(2) * d0) + (T(1) + T(KAP)) * mm2(d0, T(2) * dm));
+  return v0 + muscl_pair(dm, d0);
 }
 template <typename T>
 __device__ __forceinline__ T muscl_r(T v0, T vp1, T vp2) {
   T dp = vp2 - vp1;
   T d0 = vp1 - v0;
-  return vp1 - T(0.25) * ((T(1) - T(KAP)) * mm2(dp, T(2) * d0) + (T(1) + T(KAP)) * mm2(d0, T(2) * dp));
+  return vp1 - muscl_pair(dp, d0);
 }
 
 // ---------------------------------------------- characteristic basis -------
