@@ -54,12 +54,14 @@ template <> __device__ __forceinline__ float wmax<float>(float a, float b) { ret
 template <typename T> __device__ __forceinline__ T msign(T x) { return wsign(x); }
 template <> __device__ __forceinline__ float msign<float>(float x) { return __builtin_copysignf(1.0f, x); }
 
-// Division, square root and tanh by arithmetic type.  FP64 keeps IEEE
-// correctly rounded operations (the parity path).  The FP32 face arithmetic
-// of the mixed-precision mode uses the hardware approximations (v_rcp_f32,
-// v_sqrt_f32 and an exp-based tanh, about 1-2 ulp): on CDNA4 non-packed
-// FP32 issues at the FP64 rate, so the mixed mode gains only by dropping the
+// Division, square root and tanh by arithmetic type.  The FP64 kernels keep
+// the correctly rounded results (the parity path), through the shortened
+// sequences below where the operands allow.  The FP32 face arithmetic of the
+// mixed-precision mode uses the hardware approximations (v_rcp_f32,
+// v_sqrt_f32 and an exp-based tanh, about 1-2 ulp): on CDNA4 non-packed FP32
+// issues at the FP64 rate, so the mixed mode gains only by dropping the
 // multi-instruction IEEE sequences.
+
 // a / b from y = RN(1/b) with one correction step (Markstein 1990; Muller et
 // al., Handbook of Floating-Point Arithmetic, the FMA-based division
 // theorem): q = RN(a y), r = a - b q (exact with an FMA), RN(q + r y) is the
@@ -118,8 +120,8 @@ __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amd
 // seven FMAs) without its scaling of arguments below 2^-767 and its select
 // for zero and +inf, which act only outside this argument range: bitwise
 // sqrt(x) for finite x >= 2^-767; NaN stays NaN (+inf would give NaN, not
-// +inf, but the argument is finite in any admissible state).  The mixed unit keeps the
-// library sqrt for its few FP64 square roots.
+// +inf, but the argument is finite in any admissible state).  The mixed unit
+// keeps the library sqrt for its few FP64 square roots.
 __device__ __forceinline__ double sq(double x) {
 #ifdef WAMP_MIXED_TU
   return sqrt(x);
@@ -143,7 +145,6 @@ __device__ __forceinline__ float th(float x) {
   float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
   return x < 0.0f ? -r : r;
 }
-
 
 // Face plane m at cell offset c in either storage of Faces.  The limiter
 // kernels take the storage as a template argument (F32: mixed storage).
