@@ -61,8 +61,10 @@ typedef struct wamp_scheme {
 #define WAMP_PREC_MIXED 1
 
 /* Create a context for one grid/EOS/scheme; allocates the step's scratch
-   (face fluxes, two stage buffers, dt accumulator) on the current device.
-   Replaces main()'s allocation of the 47 work arrays, SC:1839-1866. */
+   (face fluxes, two stage buffers, dt accumulator) on the current device:
+   40 FP64 planes of the padded grid, or in WAMP_PREC_MIXED 16 FP64 planes
+   and 24 FP32 face planes.  Replaces main()'s allocation of the 47 work
+   arrays, SC:1839-1866. */
 int wamp_ctx_create(const wamp_grid *grid, const wamp_eos *eos, const wamp_scheme *scheme,
                     int32_t precision, wamp_ctx **out);
 int wamp_ctx_destroy(wamp_ctx *ctx);
