@@ -50,3 +50,23 @@ def test_mixed_is_not_fp64(gpu_ok):
         out.append(s.state())
     d = np.abs(out[0] - out[1]).max()
     assert 0.0 < d < 1e-3
+
+
+def test_scratch_holds_fp32_face_planes_in_mixed_precision(gpu_ok):
+    """FP64: 28 face + 12 stage planes; mixed: 4 FP64 energy-flux planes and 24
+    FP32 face planes + 12 stage planes (plus the dt accumulator)."""
+    from wamp import cases
+    from wamp.config import PREC_FP64, PREC_MIXED, SCHEME_MP5
+    from wamp.solver import Solver
+
+    cfg, _ = cases.shock_cylinder(160, 80, scheme=SCHEME_MP5)
+    got = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        s = Solver(dataclasses.replace(cfg, precision=prec))
+        plane = cfg.grid.nxg * s.ld
+        got[prec] = (s.scratch_bytes(), plane)
+        s.close()
+    b64, plane = got[PREC_FP64]
+    bmix, _ = got[PREC_MIXED]
+    assert b64 == (40 * plane + 32) * 8
+    assert bmix == 4 * plane * 8 + 24 * plane * 4 + (12 * plane + 32) * 8
