@@ -289,8 +289,21 @@ constexpr int LC_U234 = 31;  // U2 U3 U4, 2 row slots
 constexpr int LU_CARRY = 37;
 constexpr int LU_WAVES = 4;  // waves per limiter block
 
-__device__ __forceinline__ double lane_up(double v) { return __shfl_up(v, 1, MW); }
-__device__ __forceinline__ double lane_down(double v) { return __shfl_down(v, 1, MW); }
+// Neighbouring lane's value (lane_up: lane i gets lane i-1; lane_down: lane i
+// gets lane i+1) by a DPP wave shift of each 32-bit half: one VALU move per
+// half instead of an LDS permute.  Lane 0 (lane_up) and lane 63 (lane_down)
+// get 0; they are halo lanes, whose thetas never reach an output lane.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double lane_shift(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_up(double v) { return lane_shift<DPP_WAVE_SHR1>(v); }
+__device__ __forceinline__ double lane_down(double v) { return lane_shift<DPP_WAVE_SHL1>(v); }
 
 // f <- L + theta_k (f - L) for k < nb (SC:1506-1512 once per limiter stage)
 __device__ __forceinline__ double blend3(double H, double L, const double* th, int nb) {
