@@ -1,6 +1,7 @@
-"""The ctypes binding shown in INTEGRATION.md (the stub a maintainer would add
-beside the reference script) loads libwamp.so and declares every entry point
-it uses -- executed here on the CPU (no GPU call is made)."""
+"""The ctypes binding printed in the repository's integration guide (the stub a
+maintainer would add beside the reference script, `wamp_binding.py`) loads
+libwamp.so and declares every entry point it uses -- executed here on the CPU
+(no GPU call is made)."""
 import os
 import re
 
@@ -10,11 +11,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "wave-multiphase_amd", "csrc", "libwamp.so")
 
 
+GUIDE = os.path.join(ROOT, "INTEGRATION.md")
+
+
 def _binding_block():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    text = open(GUIDE).read()
     blocks = re.findall(r"```python\n(.*?)```", text, re.S)
     stub = [b for b in blocks if "wamp_binding.py" in b]
-    assert stub, "INTEGRATION.md lost its binding block"
+    assert stub, "the integration guide lost its wamp_binding.py block"
     return stub[0]
 
 
@@ -22,7 +26,7 @@ def _binding_block():
 def test_binding_block_executes_against_the_library(monkeypatch):
     monkeypatch.chdir(ROOT)
     ns = {}
-    exec(compile(_binding_block(), "INTEGRATION.md", "exec"), ns)
+    exec(compile(_binding_block(), GUIDE, "exec"), ns)
     L = ns["L"]
     for name in ("wamp_ctx_create", "wamp_apply_bc", "wamp_enforce_admissible", "wamp_cons_to_prim",
                  "wamp_prepare", "wamp_compute_fluxes", "wamp_limit_update", "wamp_get_dt", "wamp_step"):
