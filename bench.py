@@ -114,18 +114,50 @@ def mixed_leg(cfg, U0, warmup: int, steps: int):
     }
 
 
-def batched_leg(n_cases: int, steps: int, seed: int = 4):
-    """64 seeded uneven shock-droplet cases: plain FP64 one case at a time versus
-    one batch in mixed precision (autotuned, step graph-captured).  Both run
-    `steps` steps per case; cases/hour is normalised to 500-step cases."""
+def _timed_batch(work, precision: int, steps: int):
+    """One batch of `work` in `precision`: autotuned, step graph-captured, one
+    warm-up replay from the initial states; returns (seconds for `steps` steps,
+    tuning, graph steps, the batch with its final states)."""
     import dataclasses
 
+    import torch
+
+    from wamp.batch import Batch
+    from wamp.config import SCHEME_MP5
+
+    b = Batch(SCHEME_MP5, precision, max_cases=len(work))
+    for k, (cfg, U0) in enumerate(work):
+        b.set_case(k, dataclasses.replace(cfg, precision=precision), U0)
+    b.commit()
+    b.prepare()
+    tuning = b.autotune(steps=2)
+    gsteps = 10 if steps % 10 == 0 else 1
+    b.capture(gsteps)
+    snap = {k: v.clone() for k, v in b.states.items()}
+    b.step()  # first replay (warm-up), then back to the initial states
+    for k, v in snap.items():
+        b.states[k].copy_(v)
+    b.prepare()
+    del snap
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        done += b.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, tuning, gsteps, b
+
+
+def batched_leg(n_cases: int, steps: int, seed: int = 4):
+    """64 seeded uneven shock-droplet cases: plain FP64 one case at a time versus
+    one batch in mixed precision (autotuned, step graph-captured), and, to
+    separate batching from precision, the same batch in FP64.  All run `steps`
+    steps per case; cases/hour is normalised to 500-step cases."""
     import numpy as np
     import torch
 
     from wamp import accuracy, cases
-    from wamp.batch import Batch
-    from wamp.config import PREC_MIXED, SCHEME_MP5
+    from wamp.config import PREC_FP64, PREC_MIXED, SCHEME_MP5
     from wamp.solver import Solver
 
     rng = np.random.default_rng(seed)
@@ -147,28 +179,12 @@ def batched_leg(n_cases: int, steps: int, seed: int = 4):
         s.close()
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    # batched mixed precision
-    b = Batch(SCHEME_MP5, PREC_MIXED, max_cases=n_cases)
-    for k, (cfg, U0) in enumerate(work):
-        b.set_case(k, dataclasses.replace(cfg, precision=PREC_MIXED), U0)
-    b.commit()
-    b.prepare()
-    tuning = b.autotune(steps=2)
-    gsteps = 10 if steps % 10 == 0 else 1
-    b.capture(gsteps)
-    snap = {k: v.clone() for k, v in b.states.items()}
-    b.step()  # first replay (warm-up), then back to the initial states
-    for k, v in snap.items():
-        b.states[k].copy_(v)
-    b.prepare()
-    del snap
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        done += b.step()
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
+    t_b64, _, _, b = _timed_batch(work, PREC_FP64, steps)
+    same64 = all(np.array_equal(b.state(k)[:, cfg.grid.interior()[0], cfg.grid.interior()[1]],
+                                plain_states[k][:, cfg.grid.interior()[0], cfg.grid.interior()[1]])
+                 for k, (cfg, _) in enumerate(work[:len(plain_states)]))
+    b.close()
+    t_batch, tuning, gsteps, b = _timed_batch(work, PREC_MIXED, steps)
     spot = []
     for k in range(min(4, n_cases)):
         cfg = work[k][0]
@@ -185,6 +201,9 @@ def batched_leg(n_cases: int, steps: int, seed: int = 4):
         "speedup": round(t_plain / t_batch, 3),
         "plain_fp64_gcell_updates_per_s": round(cells * steps / t_plain / 1e9, 4),
         "batched_mixed_gcell_updates_per_s": round(cells * steps / t_batch / 1e9, 4),
+        "batched_fp64_cases_per_hour": round(n_cases / (t_b64 * scale) * 3600.0, 2),
+        "batching_speedup_fp64": round(t_plain / t_b64, 3),
+        "batched_fp64_equals_plain_bitwise": bool(same64),
         "tuning": list(tuning.as_tuple()),
         "graph_steps": gsteps,
         "spot_check_vs_fp64": {
