@@ -139,7 +139,11 @@ __device__ __forceinline__ double sq(double x) {
 #endif
 }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double th(double x) { return tanh(x); }
+// FP64 tanh (THINC) out of line: the library tanh is ~170 instructions and
+// the face kernel has up to 12 THINC sites; one shared body keeps the M10
+// FP64 face kernel at 44 KB of code instead of 60 KB (the instruction cache
+// is 64 KB per CU pair).  It runs only where THINC fires.
+__device__ __noinline__ double th(double x) { return tanh(x); }
 __device__ __forceinline__ float th(float x) {
   float t = __expf(-2.0f * fabsf(x));
   float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
