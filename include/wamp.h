@@ -207,6 +207,14 @@ int64_t wamp_batch_work(const wamp_batch *b, int32_t which);
 int wamp_timing_enable(wamp_ctx *ctx, int32_t max_steps);
 int wamp_timing_read(wamp_ctx *ctx, double ms[2], int64_t launches[2]);
 
+/* Arithmetic self-test (no counterpart in the reference; for tests).  For
+   i < n: out[i] = a[i] / b[i] by the FP64 kernels' division sequence (the
+   hardware reciprocal, two Newton steps, one correction), out[n + i] = the
+   compiler's IEEE a[i] / b[i], out[2n + i] = sqrt(b[i]) by the kernels'
+   square-root sequence, out[3n + i] = the IEEE sqrt(b[i]).  a, b: n device
+   doubles; out: 4n device doubles. */
+int wamp_selftest_arith(const double *a, const double *b, double *out, int64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

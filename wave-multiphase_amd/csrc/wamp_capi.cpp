@@ -39,6 +39,8 @@ hipError_t wamp_launch_dt(const Geo*, const Eos*, const State*, double* dt_acc, 
                           double cfl, hipStream_t);
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t);
 hipError_t wamp_launch_fill(double* p, double v, hipStream_t);
+hipError_t wamp_launch_selftest_arith(const double* a, const double* b, double* out, long long n,
+                                      hipStream_t);
 hipError_t wamp_launch_prepare(const Geo*, const Eos*, const State*, int repair, hipStream_t);
 hipError_t wamp_launch_enforce(const Geo*, const Eos*, const State*, hipStream_t);
 hipError_t wamp_launch_cons_to_prim(const Geo*, const Eos*, const State*, double*, double*,
@@ -436,6 +438,14 @@ int wamp_init_triple_point(const wamp_grid* grid, const wamp_eos* eos, const wam
              ic->rho_top, ic->p_top, ic->rho_bottom, ic->p_bottom};
   const Eos e = make_eos(*eos);
   return hip_check(wamp_launch_ic_triple(&G, &e, &s, &t, (hipStream_t)stream), "ic_triple");
+}
+
+int wamp_selftest_arith(const double* a, const double* b, double* out, int64_t n, void* stream) {
+  if (n < 0 || n > (int64_t)1 << 30) return fail(-3, "n must be in [0, 2^30]");
+  if (n == 0) return 0;
+  if (!a || !b || !out) return fail(-2, "a, b and out must be non-NULL");
+  return hip_check(wamp_launch_selftest_arith(a, b, out, (long long)n, (hipStream_t)stream),
+                   "selftest_arith");
 }
 
 int wamp_timing_enable(wamp_ctx* c, int32_t max_steps) {

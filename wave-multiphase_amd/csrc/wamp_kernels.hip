@@ -682,6 +682,22 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
 }
 
 // ========================================================================
+// arithmetic self-test: the FP64 kernels' division and square-root
+// sequences beside the compiler's IEEE operations, element by element
+// ========================================================================
+__global__ void __launch_bounds__(256) k_selftest_arith(const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i], y = b[i];
+  out[i] = ddiv(x, y);
+  out[n + i] = x / y;
+  out[2 * n + i] = sq(y);
+  out[3 * n + i] = sqrt(y);
+}
+
+// ========================================================================
 // small kernels
 // ========================================================================
 constexpr int STHREADS = 256;
@@ -1039,6 +1055,13 @@ hipError_t wamp_launch_dt(const Geo* G, const Eos* e, const State* U, double* dt
 
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t st) {
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, dt_acc, dt_out, cfl);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_selftest_arith(const double* a, const double* b, double* out, long long n,
+                                      hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest_arith, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b,
+                     out, n);
   return hipGetLastError();
 }
 

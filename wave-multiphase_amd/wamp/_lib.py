@@ -86,6 +86,7 @@ SIGNATURES = [
     ("wamp_batch_work", C.c_int64, [_VP, C.c_int32]),
     ("wamp_timing_enable", C.c_int, [_VP, C.c_int32]),
     ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    ("wamp_selftest_arith", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
 ]
 
 _lib = None
