@@ -54,10 +54,30 @@ def compare(cfg: CaseConfig, U, Uref) -> dict:
     }
 
 
-def paired_run(cfg: CaseConfig, U0, steps: int, checkpoints=(), log=None) -> list:
-    """Advance one FP64 and one mixed-precision solver from the same state with
-    the FP64 run's dt sequence (so both sit at the same time); errors of the
-    mixed state against the FP64 state at each checkpoint step."""
+def perturbed(cfg: CaseConfig, U0, eps: float, seed: int = 0):
+    """U0 with the partial densities and the energy of every interior cell scaled
+    by 1 + eps or 1 - eps (seeded signs): a round-off-sized perturbation."""
+    g = cfg.grid
+    si, sj = g.interior()
+    U = np.array(U0, dtype=np.float64, copy=True)
+    rng = np.random.default_rng(seed)
+    for m in (0, 1, 4):
+        sgn = rng.choice([-1.0, 1.0], size=U[m][si, sj].shape)
+        U[m][si, sj] *= 1.0 + eps * sgn
+    return U
+
+
+def paired_run(cfg: CaseConfig, U0, steps: int, checkpoints=(), log=None, variant: str = "mixed",
+               eps: float = 2.0 ** -52, t_end: float | None = None) -> list:
+    """Advance one FP64 solver and one variant from the same state with the FP64
+    run's dt sequence (so both sit at the same time); errors of the variant's
+    state against the FP64 state at each checkpoint step.
+
+    variant "mixed": the mixed-precision solver.  variant "perturbed": FP64 again,
+    from the initial state perturbed by a relative eps per cell (`perturbed`),
+    which measures how much the flow itself amplifies round-off-sized
+    differences.  With t_end, stepping ends when the FP64 run reaches t_end
+    (the last dt clipped), or after `steps` steps, whichever comes first."""
     import dataclasses
 
     import torch
@@ -65,33 +85,44 @@ def paired_run(cfg: CaseConfig, U0, steps: int, checkpoints=(), log=None) -> lis
     from .solver import Solver
 
     a = Solver(dataclasses.replace(cfg, precision=0))
-    b = Solver(dataclasses.replace(cfg, precision=1))
+    b = Solver(dataclasses.replace(cfg, precision=1 if variant == "mixed" else 0))
+    a.set_state(U0)
+    b.set_state(U0 if variant == "mixed" else perturbed(cfg, U0, eps))
     for s in (a, b):
-        s.set_state(U0)
         s.prepare()
     a.compute_dt()
-    t = torch.zeros(1, dtype=torch.float64, device=a.device)
+    t = 0.0
     marks = sorted(set(int(c) for c in checkpoints) | {int(steps)})
     rows = []
-    for k in range(1, int(steps) + 1):
+    k = 0
+    while k < int(steps):
+        k += 1
+        last = False
+        if t_end is not None:
+            dt = float(a.dt_buf[0].item())
+            if t + dt >= t_end:
+                a.dt_buf[0].fill_(t_end - t)
+                last = True
         b.dt_buf[0].copy_(a.dt_buf[0])
-        t += a.dt_buf[0]
+        t += float(a.dt_buf[0].item())
         a.step_device()
         b.step_device()
-        if k in marks:
+        if k in marks or last:
             row = compare(cfg, b.state(), a.state())
-            row.update(step=k, t=float(t.item()))
+            row.update(step=k, t=t)
             rows.append(row)
             if log:
-                log(f"  step {k}: rho {row['rho']:.3e}  p {row['p']:.3e}  alpha1 {row['alpha1']:.3e}  "
-                    f"centroid {row['centroid_shift_cells']:.3e} cells")
+                log(f"  step {k} (t {t:.4g}): rho {row['rho']:.3e}  p {row['p']:.3e}  "
+                    f"alpha1 {row['alpha1']:.3e}  centroid {row['centroid_shift_cells']:.3e} cells")
+        if last:
+            break
     a.close()
     b.close()
     return rows
 
 
 def main(argv=None) -> int:
-    """Mixed-vs-FP64 error report for one named case (JSON on stdout)."""
+    """Mixed-vs-FP64 (or perturbed-vs-FP64) error report for one named case (JSON on stdout)."""
     import argparse
     import json
     import sys
@@ -106,14 +137,23 @@ def main(argv=None) -> int:
     ap.add_argument("--scheme", choices=sorted(SCHEMES), default=None)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--checkpoints", type=str, default="1,10,100")
+    ap.add_argument("--variant", choices=("mixed", "perturbed"), default="mixed",
+                    help="compare mixed precision, or FP64 from a perturbed initial state")
+    ap.add_argument("--eps", type=float, default=2.0 ** -52,
+                    help="relative perturbation of the initial state (variant perturbed)")
+    ap.add_argument("--t-end", type=float, default=None)
     args = ap.parse_args(argv)
     make, (nx0, ny0), scheme0, _ = cli.CASES[args.case]
     scheme = args.scheme or scheme0
     cfg, U0 = make(args.nx or nx0, args.ny or ny0, scheme=SCHEMES[scheme])
     cps = [int(c) for c in args.checkpoints.split(",") if c.strip()]
-    rows = paired_run(cfg, U0, args.steps, cps, log=lambda m: print(m, file=sys.stderr, flush=True))
-    print(json.dumps({"case": cfg.name, "scheme": scheme, "grid": [cfg.grid.nx, cfg.grid.ny],
-                      "dt_sequence": "fp64 run's", "errors": rows}), flush=True)
+    rows = paired_run(cfg, U0, args.steps, cps, log=lambda m: print(m, file=sys.stderr, flush=True),
+                      variant=args.variant, eps=args.eps, t_end=args.t_end)
+    out = {"case": cfg.name, "scheme": scheme, "grid": [cfg.grid.nx, cfg.grid.ny],
+           "variant": args.variant, "dt_sequence": "fp64 run's", "errors": rows}
+    if args.variant == "perturbed":
+        out["eps"] = args.eps
+    print(json.dumps(out), flush=True)
     return 0
 
 
