@@ -527,8 +527,15 @@ __device__ __forceinline__ void wong_lower(const double* b, double* s, double bv
 #pragma unroll
   for (int m = 0; m < 6; ++m) s[m] = b[m] + theta * (s[m] - b[m]);
 }
-// pp_limit_reconstructed_state SC:217-240 (in place on hi; base unchanged)
-__device__ __forceinline__ void pp_limit_state(const Eos& e, double* hi, const double* base) {
+// The M10 face positivity sequence for one side (M10:1007-1021), in place on
+// the reconstructed state hi, with base the first-order state of that side:
+// pp_limit_reconstructed_state (SC:217-240), the fallbacks of the partial
+// densities, alpha1 and p to base, and pp_floor_pressure_state (SC:180-187).
+// rho*c~^2 of the state leaving the limiter's hard-stop check is carried into
+// the later checks and recomputed only in the (rare) branches that change p
+// or alpha1, so every check sees the value a fresh rho_ctilde2_sc of the
+// current state gives.
+__device__ __forceinline__ void face_state_pp(const Eos& e, double* hi, const double* base) {
   double b[6];
 #pragma unroll
   for (int m = 0; m < 6; ++m) b[m] = base[m];
@@ -546,18 +553,31 @@ __device__ __forceinline__ void pp_limit_state(const Eos& e, double* hi, const d
 #pragma unroll
     for (int m = 0; m < 6; ++m) hi[m] = b[m];
   }
-  if (rho_ct2_sc(e, hi) < PP_HS_CTILDE || isnan_(hi[4])) {
+  double v = rho_ct2_sc(e, hi);
+  if (v < PP_HS_CTILDE || isnan_(hi[4])) {
 #pragma unroll
     for (int m = 0; m < 6; ++m) hi[m] = b[m];
+    v = rho_ct2_sc(e, hi);
   }
-}
-// pp_floor_pressure_state SC:180-187 (in place)
-__device__ __forceinline__ void floor_pressure(const Eos& e, double* s) {
-  double a1 = clamp01_pp(s[5]);
-  double g, pi;
-  mix_g_pi(e, a1, g, pi);
-  if (isnan_(s[4]) || rho_ct2_sc(e, s) < PP_CTILDE) s[4] = PP_CTILDE * (g - 1.0) - pi;
-  s[5] = a1;
+  // fallbacks to the first-order state (M10:1011-1015)
+  if (hi[0] < 0.0 || isnan_(hi[0])) hi[0] = base[0];
+  if (hi[1] < 0.0 || isnan_(hi[1])) hi[1] = base[1];
+  if (hi[5] < 0.0 || hi[5] > 1.0) {
+    hi[5] = base[5];
+    v = rho_ct2_sc(e, hi);
+  }
+  if (v < PP_CTILDE || isnan_(hi[4])) {
+    hi[4] = base[4];
+    v = rho_ct2_sc(e, hi);
+  }
+  // pp_floor_pressure_state
+  const double a1 = clamp01_pp(hi[5]);
+  if (isnan_(hi[4]) || v < PP_CTILDE) {
+    double g, pi;
+    mix_g_pi(e, a1, g, pi);
+    hi[4] = PP_CTILDE * (g - 1.0) - pi;
+  }
+  hi[5] = a1;
 }
 
 // ---------------------------------------------------------------- HLLC -----

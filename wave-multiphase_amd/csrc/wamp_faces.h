@@ -157,6 +157,7 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, i
   to_phys<DIR>(B, w[2], phfL);
   to_phys<DIR>(B, w[3], phfR);
 
+  const bool cbad = ca < T(0) || isnan_(ca);
   if (FACEPP == 0) {  // SC:1103-1111
     if (phL[0] < T(0) || isnan_(phL[0])) phL[0] = phfL[0];
     if (phL[1] < T(0) || isnan_(phL[1])) phL[1] = phfL[1];
@@ -166,49 +167,44 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, i
     if (phR[1] < T(0) || isnan_(phR[1])) phR[1] = phfR[1];
     if (phR[5] < T(0) || phR[5] > T(1)) phR[5] = phfR[5];
     if (isnan_(phR[4])) phR[4] = phfR[4];
+    if (cbad) {  // SC:1113-1116
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        phL[m] = phfL[m];
+        phR[m] = phfR[m];
+      }
+    }
   } else if constexpr (sizeof(T) == sizeof(double)) {  // M10:1007-1021
-    pp_limit_state(e, phL, phfL);
-    pp_limit_state(e, phR, phfR);
-    if (phL[0] < 0.0 || isnan_(phL[0])) phL[0] = phfL[0];
-    if (phL[1] < 0.0 || isnan_(phL[1])) phL[1] = phfL[1];
-    if (phL[5] < 0.0 || phL[5] > 1.0) phL[5] = phfL[5];
-    if (rho_ct2_sc(e, phL) < PP_CTILDE || isnan_(phL[4])) phL[4] = phfL[4];
-    floor_pressure(e, phL);
-    if (phR[0] < 0.0 || isnan_(phR[0])) phR[0] = phfR[0];
-    if (phR[1] < 0.0 || isnan_(phR[1])) phR[1] = phfR[1];
-    if (phR[5] < 0.0 || phR[5] > 1.0) phR[5] = phfR[5];
-    if (rho_ct2_sc(e, phR) < PP_CTILDE || isnan_(phR[4])) phR[4] = phfR[4];
-    floor_pressure(e, phR);
+    // each side's sound-speed fallback (M10:1023-1025) right after its own
+    // sequence, so the first-order state of L is dead while R is limited
+    face_state_pp(e, phL, phfL);
+    if (cbad) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) phL[m] = phfL[m];
+    }
+    face_state_pp(e, phR, phfR);
+    if (cbad) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) phR[m] = phfR[m];
+    }
   } else {  // the same on FP64 copies: the thresholds are always applied in FP64
-    double dL[6], dR[6], dfL[6], dfR[6];
+    double d[6], df[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      dL[m] = phL[m]; dR[m] = phR[m]; dfL[m] = phfL[m]; dfR[m] = phfR[m];
+      d[m] = phL[m];
+      df[m] = phfL[m];
     }
-    pp_limit_state(e, dL, dfL);
-    pp_limit_state(e, dR, dfR);
-    if (dL[0] < 0.0 || isnan_(dL[0])) dL[0] = dfL[0];
-    if (dL[1] < 0.0 || isnan_(dL[1])) dL[1] = dfL[1];
-    if (dL[5] < 0.0 || dL[5] > 1.0) dL[5] = dfL[5];
-    if (rho_ct2_sc(e, dL) < PP_CTILDE || isnan_(dL[4])) dL[4] = dfL[4];
-    floor_pressure(e, dL);
-    if (dR[0] < 0.0 || isnan_(dR[0])) dR[0] = dfR[0];
-    if (dR[1] < 0.0 || isnan_(dR[1])) dR[1] = dfR[1];
-    if (dR[5] < 0.0 || dR[5] > 1.0) dR[5] = dfR[5];
-    if (rho_ct2_sc(e, dR) < PP_CTILDE || isnan_(dR[4])) dR[4] = dfR[4];
-    floor_pressure(e, dR);
+    face_state_pp(e, d, df);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) phL[m] = cbad ? phfL[m] : (T)d[m];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
-      phL[m] = (T)dL[m];
-      phR[m] = (T)dR[m];
+      d[m] = phR[m];
+      df[m] = phfR[m];
     }
-  }
-  if (ca < T(0) || isnan_(ca)) {
+    face_state_pp(e, d, df);
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      phL[m] = phfL[m];
-      phR[m] = phfR[m];
-    }
+    for (int m = 0; m < 6; ++m) phR[m] = cbad ? phfR[m] : (T)d[m];
   }
   // primitives for HLLC, SC:1118-1125
   T rhoL = phL[0] + phL[1], rhoR = phR[0] + phR[1];
