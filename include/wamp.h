@@ -215,6 +215,38 @@ int wamp_timing_read(wamp_ctx *ctx, double ms[2], int64_t launches[2]);
    doubles; out: 4n device doubles. */
 int wamp_selftest_arith(const double *a, const double *b, double *out, int64_t n, void *stream);
 
+/* ---- Three-dimensional extension (no counterpart in the reference, whose
+   scripts are 2D; its README mentions 3D runs whose code it does not hold).
+   The same step with a z-sweep and the 7-variable state [a1*rho1, a2*rho2,
+   rho*u, rho*v, rho*w, E, a1]; each array has shape (nx+2g, ny+2g, ldz)
+   with z contiguous, interior [g, g+nx) x [g, g+ny) x [g, g+nz).  A
+   z-invariant state with w = 0 takes exactly the 2D step of its (x, y)
+   slice, and a y-invariant state with v = 0 exactly the 2D step of its
+   (x, z) slice, for the same dt.  FP64; schemes SC, M10 and MP5. */
+typedef struct wamp3_ctx wamp3_ctx;
+
+/* bc: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi, as in wamp_grid. */
+typedef struct wamp_grid3 {
+    int32_t nx, ny, nz, g, ldz;
+    double dx, dy, dz;
+    int32_t bc[6];
+} wamp_grid3;
+
+/* Allocates 68 FP64 planes of the padded grid: 48 face planes, two stage
+   buffers, pressure / entropy / Ducros planes, three limiter theta planes. */
+int wamp3_ctx_create(const wamp_grid3 *grid, const wamp_eos *eos, const wamp_scheme *scheme,
+                     wamp3_ctx **out);
+int wamp3_ctx_destroy(wamp3_ctx *ctx);
+int64_t wamp3_ctx_scratch_bytes(const wamp3_ctx *ctx);
+/* Interior repair + a1 clamp, as wamp_prepare. */
+int wamp3_prepare(wamp3_ctx *ctx, double *const U[7], void *stream);
+/* *dt_dev = cfl * min over interior cells of 1 / (1/sx + 1/sy + 1/sz),
+   s_d = d_d / (|u_d| + c): the 2D sx*sy/(sx+sy) of get_dt with z added. */
+int wamp3_get_dt(wamp3_ctx *ctx, double *const U[7], double *dt_dev, void *stream);
+/* One SSP-RK3 step in place on U, as wamp_step. */
+int wamp3_step(wamp3_ctx *ctx, double *const U[7], const double *dt_dev, double *dt_next_dev,
+               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

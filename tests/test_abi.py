@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "wamp.h")
 def declared_symbols():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(wamp_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(wamp3?_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -74,3 +74,37 @@ def test_null_context_is_an_error():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_lib.WampError, match="not found"):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+def _grid3(**kw):
+    g = dict(nx=8, ny=6, nz=4, g=5, ldz=14, dx=0.1, dy=0.1, dz=0.1)
+    g.update(kw)
+    return _lib.WampGrid3(g["nx"], g["ny"], g["nz"], g["g"], g["ldz"], g["dx"], g["dy"], g["dz"],
+                          (C.c_int32 * 6)(0, 0, 1, 1, 1, 1))
+
+
+@pytest.mark.parametrize("kw, scheme, msg", [
+    (dict(nz=0), (0, 1, 0, 0), "nx, ny, nz"),
+    (dict(g=3), (0, 1, 0, 0), "g must be >= 4"),
+    (dict(ldz=13), (0, 1, 0, 0), "ldz"),
+    (dict(dz=0.0), (0, 1, 0, 0), "dx, dy, dz"),
+    ({}, (1, 0, 0, 0), "not built"),  # MP5 without central6: no 3D instantiation
+])
+def test_ctx3_create_rejects_bad_grids_and_schemes(kw, scheme, msg):
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    eos = _lib.WampEos(4.4, 6000.0, 1.4, 0.0)
+    sch = _lib.WampScheme(*scheme, 1, 0.4)
+    rc = lib.wamp3_ctx_create(C.byref(_grid3(**kw)), C.byref(eos), C.byref(sch), C.byref(ctx))
+    assert rc < 0 and not ctx.value
+    assert msg in lib.wamp_last_error().decode()
+
+
+def test_3d_entry_points_reject_null_contexts():
+    lib = _lib.load()
+    U = _lib.PtrArr7(*([None] * 7))
+    assert lib.wamp3_step(None, U, None, None, None) < 0
+    assert lib.wamp3_get_dt(None, U, None, None) < 0
+    assert lib.wamp3_prepare(None, U, None) < 0
+    assert lib.wamp3_ctx_scratch_bytes(None) == -1
+    assert lib.wamp3_ctx_destroy(None) == 0

@@ -61,6 +61,18 @@ hipError_t wamp_launch_batch_finalize(const CaseDesc* cs, int ncase, const Batch
                                       hipStream_t st);
 hipError_t wamp_launch_batch_dt0(const Geo* G, const Eos* e, const State* U, double* acc,
                                  const BatchScal* sc, int k, double cfl, hipStream_t st);
+int wamp3_scheme_available(int base, int shear, int face_pp);
+hipError_t wamp3_launch_cells(const Geo3*, const Eos*, const State3*, double* P, double* S,
+                              double* D, int shear, hipStream_t);
+hipError_t wamp3_launch_faces(const Geo3*, const Eos*, const State3*, const double* P,
+                              const double* S, const double* D, const Faces3*, int base, int shear,
+                              int vf_always, int face_pp, double t1, hipStream_t);
+hipError_t wamp3_launch_limit_update(const Geo3*, const Eos*, const State3* Un, const State3* U,
+                                     const Faces3*, double* const th[3], const State3* Uo,
+                                     double alpha, double beta, const double* dt_dev,
+                                     double* dt_acc, int repair, hipStream_t);
+hipError_t wamp3_launch_dt(const Geo3*, const Eos*, const State3*, double* dt_acc, hipStream_t);
+hipError_t wamp3_launch_prepare(const Geo3*, const Eos*, const State3*, int repair, hipStream_t);
 }
 
 struct wamp_ctx {
@@ -830,6 +842,155 @@ int wamp_batch_status(wamp_batch* b, double* t, double* dt, int32_t* done, void*
 int64_t wamp_batch_work(const wamp_batch* b, int32_t which) {
   if (!b) return -1;
   return which == 0 ? b->total_tiles : b->total_items;
+}
+
+// ---- three-dimensional extension -------------------------------------------
+// One 3D case: 48 face planes, two 7-plane stage buffers, pressure, entropy
+// and Ducros planes, three limiter theta planes and the dt accumulator, in
+// one allocation.
+struct wamp3_ctx {
+  Geo3 geo;
+  Eos eos;
+  wamp_scheme scheme;
+  double t1;
+  double* block;
+  int64_t bytes;
+  Faces3 faces;
+  State3 u1, u2;
+  double *P, *S, *D;
+  double* th[3];
+  double* dt_acc;
+};
+
+static int state3_of(double* const U[7], State3& s, const char* name) {
+  if (!U) return fail(-2, "%s: NULL array list", name);
+  for (int m = 0; m < 7; ++m) {
+    if (!U[m]) return fail(-2, "%s[%d] is NULL", name, m);
+    s.f[m] = U[m];
+  }
+  return 0;
+}
+
+int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_scheme* scheme,
+                     wamp3_ctx** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
+    return fail(-3, "nx, ny, nz must be >= 1 (got %d, %d, %d)", grid->nx, grid->ny, grid->nz);
+  if (grid->g < 4) return fail(-3, "g must be >= 4 (the face stencil reaches 4 cells), got %d", grid->g);
+  if (grid->ldz < grid->nz + 2 * grid->g)
+    return fail(-3, "ldz (%d) must be >= nz + 2g (%d)", grid->ldz, grid->nz + 2 * grid->g);
+  if (!(grid->dx > 0.0) || !(grid->dy > 0.0) || !(grid->dz > 0.0)) return fail(-3, "dx, dy, dz must be > 0");
+  const int64_t plane64 = (int64_t)(grid->nx + 2 * grid->g) * (grid->ny + 2 * grid->g) * grid->ldz;
+  if (plane64 > (int64_t)1 << 31) return fail(-3, "grid too large (padded plane > 2^31 cells)");
+  if (grid->nx + 2 * grid->g > 65535)
+    return fail(-3, "nx + 2g must be <= 65535 (one launch row per x index)");
+  for (int k = 0; k < 6; ++k)
+    if (grid->bc[k] != 0 && grid->bc[k] != 1) return fail(-3, "bc[%d] must be 0 or 1", k);
+  if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  if (scheme->vf_thinc < 0 || scheme->vf_thinc > 1) return fail(-3, "scheme.vf_thinc must be 0 or 1");
+  if (!wamp3_scheme_available(scheme->base, scheme->shear, scheme->face_pp))
+    return fail(-4, "3D: scheme (base %d, shear %d, face_pp %d) is not built; available: SC (0,1,0), "
+                "M10 (0,0,1), MP5 (1,2,0)", scheme->base, scheme->shear, scheme->face_pp);
+  if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
+
+  wamp3_ctx* c = new (std::nothrow) wamp3_ctx();
+  if (!c) return fail(-5, "host allocation failed");
+  Geo3& G = c->geo;
+  G.nx = grid->nx;
+  G.ny = grid->ny;
+  G.nz = grid->nz;
+  G.g = grid->g;
+  G.sy = grid->ldz;
+  G.sx = (long long)(grid->ny + 2 * grid->g) * grid->ldz;
+  G.dx = grid->dx;
+  G.dy = grid->dy;
+  G.dz = grid->dz;
+  for (int k = 0; k < 6; ++k) G.bc[k] = grid->bc[k];
+  c->eos = make_eos(*eos);
+  c->scheme = *scheme;
+  c->t1 = std::tanh(1.8 * 0.5);
+  const size_t plane = (size_t)plane64;
+  const size_t bytes = ((48 + 14 + 3 + 3) * plane + 32) * sizeof(double);
+  int rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(3D scratch)");
+  if (rc) { delete c; return rc; }
+  c->bytes = (int64_t)bytes;
+  double* p = c->block;
+  c->faces = Faces3{p, (long long)plane};
+  p += 48 * plane;
+  for (int m = 0; m < 7; ++m, p += plane) c->u1.f[m] = p;
+  for (int m = 0; m < 7; ++m, p += plane) c->u2.f[m] = p;
+  c->P = p; p += plane;
+  c->S = p; p += plane;
+  c->D = p; p += plane;
+  for (int s = 0; s < 3; ++s, p += plane) c->th[s] = p;
+  c->dt_acc = p;
+  rc = hip_check(wamp_launch_fill(c->dt_acc, 1.0e30, nullptr), "fill dt accumulator");
+  if (!rc) rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  if (rc) {
+    (void)hipFree(c->block);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int wamp3_ctx_destroy(wamp3_ctx* c) {
+  if (!c) return 0;
+  int rc = 0;
+  if (c->block) rc = hip_check(hipFree(c->block), "hipFree(3D scratch)");
+  delete c;
+  return rc;
+}
+
+int64_t wamp3_ctx_scratch_bytes(const wamp3_ctx* c) { return c ? c->bytes : -1; }
+
+int wamp3_prepare(wamp3_ctx* c, double* const U[7], void* stream) {
+  if (!c) return fail(-1, "NULL context");
+  State3 s;
+  if (int rc = state3_of(U, s, "U")) return rc;
+  return hip_check(wamp3_launch_prepare(&c->geo, &c->eos, &s, c->scheme.repair, (hipStream_t)stream),
+                   "3D prepare");
+}
+
+int wamp3_get_dt(wamp3_ctx* c, double* const U[7], double* dt_dev, void* stream) {
+  if (!c) return fail(-1, "NULL context");
+  State3 s;
+  if (int rc = state3_of(U, s, "U")) return rc;
+  if (!dt_dev) return fail(-2, "dt_dev is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = hip_check(wamp3_launch_dt(&c->geo, &c->eos, &s, c->dt_acc, st), "3D dt")) return rc;
+  return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_dev, c->scheme.cfl, st), "3D dt");
+}
+
+int wamp3_step(wamp3_ctx* c, double* const U[7], const double* dt_dev, double* dt_next_dev,
+               void* stream) {
+  if (!c) return fail(-1, "NULL context");
+  State3 a;
+  if (int rc = state3_of(U, a, "U")) return rc;
+  if (!dt_dev || !dt_next_dev) return fail(-2, "dt_dev and dt_next_dev must be non-NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const State3* cur[3] = {&a, &c->u1, &c->u2};
+  const State3* dst[3] = {&c->u1, &c->u2, &a};
+  const double coef[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
+  const wamp_scheme& sc = c->scheme;
+  for (int s = 0; s < 3; ++s) {
+    if (int rc = hip_check(wamp3_launch_cells(&c->geo, &c->eos, cur[s], c->P, c->S, c->D, sc.shear, st),
+                           "3D cells"))
+      return rc;
+    if (int rc = hip_check(wamp3_launch_faces(&c->geo, &c->eos, cur[s], c->P, c->S, c->D, &c->faces,
+                                              sc.base, sc.shear, sc.vf_thinc, sc.face_pp, c->t1, st),
+                           "3D faces"))
+      return rc;
+    if (int rc = hip_check(wamp3_launch_limit_update(&c->geo, &c->eos, &a, cur[s], &c->faces, c->th,
+                                                     dst[s], coef[s][0], coef[s][1], dt_dev,
+                                                     s == 2 ? c->dt_acc : nullptr, sc.repair, st),
+                           "3D limit/update"))
+      return rc;
+  }
+  return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, sc.cfl, st), "3D dt");
 }
 
 }  // extern "C"

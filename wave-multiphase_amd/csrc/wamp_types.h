@@ -70,6 +70,28 @@ struct TripleIC {
   double rho_left, p_left, rho_top, p_top, rho_bottom, p_bottom;
 };
 
+// Three-dimensional grid: interior nx x ny x nz, g ghost layers, cell
+// (i, j, k) at i * sx + j * sy + k (k contiguous; sy >= nz + 2g,
+// sx = (ny + 2g) * sy).  bc: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi.
+struct Geo3 {
+  int nx, ny, nz, g;
+  long long sx, sy;
+  double dx, dy, dz;
+  int bc[6];
+};
+struct State3 {  // [a1 rho1, a2 rho2, rho u, rho v, rho w, E, a1]
+  double* f[7];
+};
+// The 48 face planes of a 3D step: direction d (x, y, z) holds 16 planes at
+// d * 16: the high-order fluxes of the 7 conserved variables and u*, then
+// the first-order ones; plane m starts at b + m * s.  The face d + 1/2 of
+// cell (i, j, k) is stored at the cell's index.
+struct Faces3 {
+  double* b;
+  long long s;
+  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }
+};
+
 // Per-slot device scalars of a batch.
 struct BatchScal {
   double* dt;

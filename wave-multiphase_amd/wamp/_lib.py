@@ -45,6 +45,13 @@ class WampTripleIC(C.Structure):
                                            "p_left", "rho_top", "p_top", "rho_bottom", "p_bottom")]
 
 
+class WampGrid3(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("g", C.c_int32),
+                ("ldz", C.c_int32), ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("bc", C.c_int32 * 6)]
+
+
+PtrArr7 = C.c_void_p * 7
 PtrArr6 = C.c_void_p * 6
 PtrArr3 = C.c_void_p * 3
 PtrArr28 = C.c_void_p * 28
@@ -87,6 +94,13 @@ SIGNATURES = [
     ("wamp_timing_enable", C.c_int, [_VP, C.c_int32]),
     ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("wamp_selftest_arith", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
+    ("wamp3_ctx_create", C.c_int, [C.POINTER(WampGrid3), C.POINTER(WampEos), C.POINTER(WampScheme),
+                                   C.POINTER(_VP)]),
+    ("wamp3_ctx_destroy", C.c_int, [_VP]),
+    ("wamp3_ctx_scratch_bytes", C.c_int64, [_VP]),
+    ("wamp3_prepare", C.c_int, [_VP, PtrArr7, _VP]),
+    ("wamp3_get_dt", C.c_int, [_VP, PtrArr7, _VP, _VP]),
+    ("wamp3_step", C.c_int, [_VP, PtrArr7, _VP, _VP, _VP]),
 ]
 
 _lib = None

@@ -29,6 +29,7 @@ COMMON = [
 UNITS = [
     ("wamp_kernels.hip", "-ffp-contract=off"),
     ("wamp_faces_mixed.hip", "-ffp-contract=on"),
+    ("wamp3_kernels.hip", "-ffp-contract=off"),
     ("wamp_capi.cpp", "-ffp-contract=off"),
 ]
 

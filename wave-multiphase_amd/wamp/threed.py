@@ -1,0 +1,250 @@
+"""Three-dimensional extension: grid, cases and the resident solver over
+libwamp's wamp3_* entry points (include/wamp.h).
+
+The reference's scripts are 2D (SC, M10); this is the same step with a
+z-sweep and the 7-variable state [a1 rho1, a2 rho2, rho u, rho v, rho w, E,
+a1], laid out (7, nx+2g, ny+2g, ldz) with z contiguous.  `from_2d_xy` and
+`from_2d_xz` embed a 2D state as a z- or y-invariant 3D one; the 3D step
+of such a state equals the 2D step of the slice exactly (for the same dt),
+which is how the 3D kernels are pinned to the 2D path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .cases import ALPHA_CLIP, WATER_ND, rankine_hugoniot, _smeared
+from .config import (BC_SLIP_WALL, BC_ZERO_GRADIENT, EOS, G_DEFAULT, SCHEME_M10, CaseConfig,
+                     Grid, Scheme)
+
+# conserved component of the 3D state holding each 2D one (ar1, ar2, ru, rv, E, a1)
+XY_MAP = (0, 1, 2, 3, 5, 6)   # z-invariant embedding: 2D (x, y) -> 3D (x, y)
+XZ_MAP = (0, 1, 2, 4, 5, 6)   # y-invariant embedding: 2D (x, y) -> 3D (x, z), 2D v -> w
+
+
+@dataclass(frozen=True)
+class Grid3:
+    """Uniform cell-centred 3D grid with g ghost layers on every side."""
+
+    nx: int
+    ny: int
+    nz: int
+    xmin: float = 0.0
+    xmax: float = 1.0
+    ymin: float = 0.0
+    ymax: float = 1.0
+    zmin: float = 0.0
+    zmax: float = 1.0
+    g: int = G_DEFAULT
+    # x-lo, x-hi, y-lo, y-hi, z-lo, z-hi
+    bc: tuple = (BC_ZERO_GRADIENT, BC_ZERO_GRADIENT, BC_SLIP_WALL, BC_SLIP_WALL,
+                 BC_SLIP_WALL, BC_SLIP_WALL)
+
+    @property
+    def nxg(self) -> int:
+        return self.nx + 2 * self.g
+
+    @property
+    def nyg(self) -> int:
+        return self.ny + 2 * self.g
+
+    @property
+    def nzg(self) -> int:
+        return self.nz + 2 * self.g
+
+    @property
+    def dx(self) -> float:
+        return (self.xmax - self.xmin) / self.nx
+
+    @property
+    def dy(self) -> float:
+        return (self.ymax - self.ymin) / self.ny
+
+    @property
+    def dz(self) -> float:
+        return (self.zmax - self.zmin) / self.nz
+
+    def centres(self, axis: int):
+        n, lo, d = [(self.nxg, self.xmin, self.dx), (self.nyg, self.ymin, self.dy),
+                    (self.nzg, self.zmin, self.dz)][axis]
+        return lo + (np.arange(n, dtype=np.float64) - self.g + 0.5) * d
+
+    def interior(self):
+        return (slice(self.g, self.g + self.nx), slice(self.g, self.g + self.ny),
+                slice(self.g, self.g + self.nz))
+
+
+@dataclass
+class Case3Config:
+    grid: Grid3
+    eos: EOS = field(default_factory=lambda: WATER_ND)
+    scheme: Scheme = SCHEME_M10
+    name: str = "case3"
+
+
+def conserved3(eos: EOS, a1, rho1, rho2, u, v, w, p):
+    """[ar1, ar2, ru, rv, rw, E, a1], E as prim_to_energy (SC:105-110) with w."""
+    ar1 = a1 * rho1
+    ar2 = (1.0 - a1) * rho2
+    rho = ar1 + ar2
+    a2 = 1.0 - a1
+    gb = a1 / (eos.gamma1 - 1.0) + a2 / (eos.gamma2 - 1.0)
+    pb = a1 * eos.gamma1 * eos.pinf1 / (eos.gamma1 - 1.0) + a2 * eos.gamma2 * eos.pinf2 / (eos.gamma2 - 1.0)
+    E = gb * p + pb + 0.5 * rho * ((u * u + v * v) + w * w)
+    return np.stack([ar1, ar2, rho * u, rho * v, rho * w, E, a1]).astype(np.float64)
+
+
+def shock_sphere(n: int = 128, scheme: Scheme = SCHEME_M10, mach: float = 2.4):
+    """Shock at x = 0.4 moving +x through air into a water sphere r = 0.15 at
+    (0.75, 0.5, 0.5); domain [0, 2] x [0, 1] x [0, 1] on 2n x n x n cells."""
+    grid = Grid3(2 * n, n, n, 0.0, 2.0, 0.0, 1.0, 0.0, 1.0)
+    cfg = Case3Config(grid, WATER_ND, scheme, name=f"shock_sphere_{2 * n}x{n}x{n}")
+    x = grid.centres(0)[:, None, None]
+    y = grid.centres(1)[None, :, None]
+    z = grid.centres(2)[None, None, :]
+    d = np.sqrt((x - 0.75) ** 2 + ((y - 0.5) ** 2 + (z - 0.5) ** 2))  # exactly y <-> z symmetric
+    a1 = np.clip(_smeared(d, 0.15, 1.5 * grid.dx), ALPHA_CLIP, 1.0 - ALPHA_CLIP)
+    rho_s, p_s, u_s = rankine_hugoniot(mach, WATER_ND.gamma2, 1.0, 1.0)
+    post = np.broadcast_to(x < 0.4, a1.shape)
+    rho2 = np.where(post, rho_s, 1.0)
+    p = np.where(post, p_s, 1.0)
+    u = np.where(post, u_s, 0.0)
+    zero = np.zeros_like(u)
+    U = conserved3(WATER_ND, a1, np.full_like(a1, 1000.0), rho2, u, zero, zero, p)
+    out = np.zeros_like(U)
+    si, sj, sk = grid.interior()
+    out[:, si, sj, sk] = U[:, si, sj, sk]
+    return cfg, out
+
+
+def from_2d_xy(cfg2: CaseConfig, U2: np.ndarray, nz: int = 4, bc_z=(BC_SLIP_WALL, BC_SLIP_WALL)):
+    """The 2D state as a z-invariant 3D state with w = 0 (dz = dx)."""
+    g2: Grid = cfg2.grid
+    dz = g2.dx
+    grid = Grid3(g2.nx, g2.ny, nz, g2.xmin, g2.xmax, g2.ymin, g2.ymax, 0.0, nz * dz, g2.g,
+                 tuple(g2.bc) + tuple(bc_z))
+    U = np.zeros((7, g2.nxg, g2.nyg, grid.nzg))
+    for m2, m3 in enumerate(XY_MAP):
+        U[m3] = np.asarray(U2[m2][:, : g2.nyg])[:, :, None]
+    return Case3Config(grid, cfg2.eos, cfg2.scheme, name=cfg2.name + "_xy"), U
+
+
+def from_2d_xz(cfg2: CaseConfig, U2: np.ndarray, ny: int = 4, bc_y=(BC_SLIP_WALL, BC_SLIP_WALL)):
+    """The 2D state as a y-invariant 3D state with v = 0: 2D y becomes z and
+    2D v becomes w (dy = dx)."""
+    g2: Grid = cfg2.grid
+    dy = g2.dx
+    grid = Grid3(g2.nx, ny, g2.ny, g2.xmin, g2.xmax, 0.0, ny * dy, g2.ymin, g2.ymax, g2.g,
+                 (g2.bc[0], g2.bc[1]) + tuple(bc_y) + (g2.bc[2], g2.bc[3]))
+    U = np.zeros((7, g2.nxg, grid.nyg, g2.nyg))
+    for m2, m3 in enumerate(XZ_MAP):
+        U[m3] = np.asarray(U2[m2][:, : g2.nyg])[:, None, :]
+    return Case3Config(grid, cfg2.eos, cfg2.scheme, name=cfg2.name + "_xz"), U
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class Solver3:
+    """One resident 3D case: conserved state (7 FP64 planes), dt on the device, a
+    libwamp 3D context.  Every call enqueues on the current torch stream."""
+
+    def __init__(self, cfg: Case3Config, device: str = "cuda", ldz: int | None = None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise _lib.WampError("the HIP solver needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.L = _lib.load()
+        self.cfg = cfg
+        g = cfg.grid
+        self.ldz = int(ldz) if ldz else g.nzg
+        if self.ldz < g.nzg:
+            raise ValueError("ldz must be >= nz + 2g")
+        self.device = torch.device(device)
+        self.U = torch.zeros((7, g.nxg, g.nyg, self.ldz), dtype=torch.float64, device=self.device)
+        self.dt_buf = torch.zeros(2, dtype=torch.float64, device=self.device)
+        self._u = _lib.PtrArr7(*[self.U[m].data_ptr() for m in range(7)])
+        grid = _lib.WampGrid3(g.nx, g.ny, g.nz, g.g, self.ldz, g.dx, g.dy, g.dz,
+                              (C.c_int32 * 6)(*g.bc))
+        eos = _lib.WampEos(*cfg.eos.as_tuple())
+        s = cfg.scheme
+        sch = _lib.WampScheme(s.base, s.shear, s.vf_thinc, s.face_pp, 1 if s.repair else 0, s.cfl)
+        ctx = C.c_void_p()
+        _lib.check(self.L.wamp3_ctx_create(C.byref(grid), C.byref(eos), C.byref(sch), C.byref(ctx)),
+                   "wamp3_ctx_create")
+        self.ctx = ctx
+        self.nstep = 0
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            _lib.check(self.L.wamp3_ctx_destroy(self.ctx), "wamp3_ctx_destroy")
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return _torch().cuda.current_stream(self.device).cuda_stream
+
+    @property
+    def dt_ptr(self) -> int:
+        return self.dt_buf.data_ptr()
+
+    def scratch_bytes(self) -> int:
+        return int(self.L.wamp3_ctx_scratch_bytes(self.ctx))
+
+    def set_state(self, U):
+        g = self.cfg.grid
+        if tuple(U.shape[:3]) != (7, g.nxg, g.nyg) or U.shape[3] < g.nzg:
+            raise ValueError(f"state shape {tuple(U.shape)} != {(7, g.nxg, g.nyg, g.nzg)}")
+        torch = _torch()
+        src = U if isinstance(U, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(U, dtype=np.float64))
+        self.U[..., : g.nzg].copy_(src[..., : g.nzg])
+
+    def state(self) -> np.ndarray:
+        return self.U[..., : self.cfg.grid.nzg].cpu().numpy().copy()
+
+    def prepare(self):
+        _lib.check(self.L.wamp3_prepare(self.ctx, self._u, self.stream), "wamp3_prepare")
+
+    def compute_dt(self):
+        _lib.check(self.L.wamp3_get_dt(self.ctx, self._u, self.dt_ptr, self.stream), "wamp3_get_dt")
+
+    def get_dt(self) -> float:
+        self.compute_dt()
+        return float(self.dt_buf[0].item())
+
+    def set_dt(self, dt: float):
+        self.dt_buf[0].fill_(float(dt))
+
+    def step_device(self):
+        """One SSP-RK3 step with the device-resident dt; the next dt replaces it."""
+        _lib.check(self.L.wamp3_step(self.ctx, self._u, self.dt_ptr, self.dt_ptr, self.stream),
+                   "wamp3_step")
+        self.nstep += 1
+
+    def step_with(self, dt: float):
+        """One step with the given dt (the CFL dt of the result is discarded)."""
+        self.set_dt(dt)
+        _lib.check(self.L.wamp3_step(self.ctx, self._u, self.dt_ptr, self.dt_buf[1:].data_ptr(),
+                                     self.stream), "wamp3_step")
+        self.nstep += 1
+
+    def run(self, nsteps: int):
+        """n steps with dt from the state before each step; returns the dts."""
+        torch = _torch()
+        dts = torch.zeros(nsteps, dtype=torch.float64, device=self.device)
+        self.compute_dt()
+        for k in range(nsteps):
+            dts[k].copy_(self.dt_buf[0])
+            self.step_device()
+        return dts.cpu().numpy()
