@@ -114,6 +114,45 @@ def mixed_leg(cfg, U0, warmup: int, steps: int):
     }
 
 
+def threed_leg(n: int, warmup: int, steps: int):
+    """The 3D extension on a shock-sphere case (2n x n x n cells, M10 scheme, FP64)."""
+    import torch
+
+    from wamp import threed
+
+    cfg, U0 = threed.shock_sphere(n)
+    g = cfg.grid
+    s = threed.Solver3(cfg, device="cuda:0")
+    s.set_state(U0)
+    s.prepare()
+    s.compute_dt()
+    for _ in range(warmup):
+        s.step_device()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        s.step_device()
+    e1.record()
+    torch.cuda.synchronize()
+    t_ms = e0.elapsed_time(e1)
+    finite = bool(torch.isfinite(s.U).all().item())
+    scratch = s.scratch_bytes()
+    s.close()
+    cells = g.nx * g.ny * g.nz
+    return {
+        "workload": cfg.name,
+        "scheme": "m10",
+        "value": round(cells * steps / (t_ms / 1e3) / 1e9, 6),
+        "unit": "Gcell-updates/s",
+        "ms_per_step": round(t_ms / steps, 4),
+        "steps": steps,
+        "scratch_gb": round(scratch / 1e9, 3),
+        "state_finite": finite,
+    }
+
+
 def _timed_batch(work, precision: int, steps: int):
     """One batch of `work` in `precision`: autotuned, step graph-captured, one
     warm-up replay from the initial states; returns (seconds for `steps` steps,
@@ -350,6 +389,10 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=100, help="steps per case in the batched leg")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--threed-n", type=int, default=160,
+                    help="the 3D leg's shock-sphere case has 2n x n x n cells")
+    ap.add_argument("--threed-steps", type=int, default=20,
+                    help="timed steps of the 3D leg (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
                          "of known size, to scale hardware traffic counters)")
@@ -461,6 +504,9 @@ def main(argv=None):
         out["mixed"] = mixed_leg(cfg, U0, args.warmup, args.mixed_steps)
     s.close()
     _progress(f"headline {out['value']:.4f} Gcell-updates/s")
+    if args.threed_steps > 0:
+        out["threed"] = threed_leg(args.threed_n, 3, args.threed_steps)
+        _progress(f"3D {out['threed']['value']:.4f} Gcell-updates/s")
     if args.batch_cases > 0:
         out["batched"] = batched_leg(args.batch_cases, args.batch_steps)
         _progress(f"batched speedup {out['batched']['speedup']}")
