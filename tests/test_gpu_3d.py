@@ -15,13 +15,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _pair(scheme, embed, steps, nx=64, ny=32):
+def _pair(scheme, embed, steps, nx=64, ny=32, thick=4):
     from wamp import cases, threed
     from wamp.config import SCHEMES
     from wamp.solver import Solver
 
     cfg2, U2 = cases.shock_cylinder(nx, ny, scheme=SCHEMES[scheme])
-    cfg3, U3 = (threed.from_2d_xy if embed == "xy" else threed.from_2d_xz)(cfg2, U2)
+    cfg3, U3 = (threed.from_2d_xy if embed == "xy" else threed.from_2d_xz)(cfg2, U2, thick)
     s2 = Solver(cfg2)
     s2.set_state(U2)
     s2.prepare()
@@ -75,6 +75,29 @@ def test_one_step_reduction_on_a_ragged_grid(gpu_ok):
     cfg2, U2, cfg3, U3 = _pair("mp5", "xz", steps=1, nx=37, ny=23)
     for name, a, b in _slices(cfg2, U2, cfg3, U3, "xz"):
         assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("embed", ["xy", "xz"])
+def test_reduction_holds_for_a_single_cell_thick_slab(gpu_ok, embed):
+    """nz = 1 (or ny = 1): every neighbour along the invariant axis is a ghost."""
+    cfg2, U2, cfg3, U3 = _pair("m10", embed, steps=5, thick=1)
+    for name, a, b in _slices(cfg2, U2, cfg3, U3, embed):
+        assert np.array_equal(a, b), name
+
+
+def test_padded_z_stride_gives_the_same_result(gpu_ok):
+    from wamp import threed
+
+    cfg, U0 = threed.shock_sphere(12)
+    out = []
+    for ldz in (None, cfg.grid.nzg + 5):
+        s = threed.Solver3(cfg, ldz=ldz)
+        s.set_state(U0)
+        s.prepare()
+        s.run(5)
+        out.append(s.state())
+        s.close()
+    assert np.array_equal(out[0], out[1])
 
 
 def test_shock_sphere_is_finite_and_keeps_its_y_z_symmetry(gpu_ok):
