@@ -1,4 +1,4 @@
-"""Host driver of the HIP solver step: the MI355X counterpart of main() (SC:1828-2082).
+"""Host side of the HIP solver step: the MI355X counterpart of main() (SC:1828-2082).
 
 Device buffers are PyTorch-ROCm tensors; every kernel runs through
 libwamp.so's C-ABI on the current torch stream.  There is no host or
