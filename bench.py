@@ -115,42 +115,51 @@ def mixed_leg(cfg, U0, warmup: int, steps: int):
 
 
 def threed_leg(n: int, warmup: int, steps: int):
-    """The 3D extension on a shock-sphere case (2n x n x n cells, M10 scheme, FP64)."""
+    """The 3D extension on a shock-sphere case (2n x n x n cells, M10 scheme),
+    FP64 and FP32-mixed, the mixed run's error against the FP64 run (same dt
+    sequence) after the timed steps."""
+    import dataclasses
+
     import torch
 
     from wamp import threed
 
     cfg, U0 = threed.shock_sphere(n)
     g = cfg.grid
-    s = threed.Solver3(cfg, device="cuda:0")
-    s.set_state(U0)
-    s.prepare()
-    s.compute_dt()
-    for _ in range(warmup):
-        s.step_device()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(steps):
-        s.step_device()
-    e1.record()
-    torch.cuda.synchronize()
-    t_ms = e0.elapsed_time(e1)
-    finite = bool(torch.isfinite(s.U).all().item())
-    scratch = s.scratch_bytes()
-    s.close()
     cells = g.nx * g.ny * g.nz
-    return {
-        "workload": cfg.name,
-        "scheme": "m10",
-        "value": round(cells * steps / (t_ms / 1e3) / 1e9, 6),
-        "unit": "Gcell-updates/s",
-        "ms_per_step": round(t_ms / steps, 4),
-        "steps": steps,
-        "scratch_gb": round(scratch / 1e9, 3),
-        "state_finite": finite,
-    }
+    res, states, dts = {}, {}, None
+    for prec in (0, 1):
+        s = threed.Solver3(dataclasses.replace(cfg, precision=prec), device="cuda:0")
+        s.set_state(U0)
+        s.prepare()
+        if prec == 0:
+            s.compute_dt()
+            dts = []
+        for k in range(warmup + steps):
+            if k == warmup:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+            if prec == 0:
+                dts.append(s.dt_buf[0].clone())
+                s.step_device()
+            else:
+                s.dt_buf[0].copy_(dts[k])
+                s.step_with_device_dt()
+        e1.record()
+        torch.cuda.synchronize()
+        t_ms = e0.elapsed_time(e1)
+        res[prec] = {"value": round(cells * steps / (t_ms / 1e3) / 1e9, 6),
+                     "ms_per_step": round(t_ms / steps, 4),
+                     "scratch_gb": round(s.scratch_bytes() / 1e9, 3),
+                     "state_finite": bool(torch.isfinite(s.U).all().item())}
+        states[prec] = s.state()
+        s.close()
+    err = threed.compare3(cfg, states[1], states[0])
+    return dict(res[0], workload=cfg.name, scheme="m10", unit="Gcell-updates/s", steps=steps,
+                mixed=dict(res[1], vs_fp64_after_steps={k: float(v) for k, v in err.items()
+                                                        if k != "finite"}))
 
 
 def _timed_batch(work, precision: int, steps: int):

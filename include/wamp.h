@@ -222,7 +222,8 @@ int wamp_selftest_arith(const double *a, const double *b, double *out, int64_t n
    with z contiguous, interior [g, g+nx) x [g, g+ny) x [g, g+nz).  A
    z-invariant state with w = 0 takes exactly the 2D step of its (x, y)
    slice, and a y-invariant state with v = 0 exactly the 2D step of its
-   (x, z) slice, for the same dt.  FP64; schemes SC, M10 and MP5. */
+   (x, z) slice, for the same dt.  FP64 or FP32-mixed; schemes SC, M10 and
+   MP5. */
 typedef struct wamp3_ctx wamp3_ctx;
 
 /* bc: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi, as in wamp_grid. */
@@ -233,9 +234,13 @@ typedef struct wamp_grid3 {
 } wamp_grid3;
 
 /* Allocates 68 FP64 planes of the padded grid: 48 face planes, two stage
-   buffers, pressure / entropy / Ducros planes, three limiter theta planes. */
+   buffers, pressure / entropy / Ducros planes, three limiter theta planes.
+   precision WAMP_PREC_MIXED: FP32 projection, reconstruction and HLLC in the
+   face kernels (energy flux, pressure, indicators and limiters FP64), and
+   the face planes as 6 FP64 energy planes + 42 FP32 planes (47 FP64-plane
+   equivalents in all). */
 int wamp3_ctx_create(const wamp_grid3 *grid, const wamp_eos *eos, const wamp_scheme *scheme,
-                     wamp3_ctx **out);
+                     int32_t precision, wamp3_ctx **out);
 int wamp3_ctx_destroy(wamp3_ctx *ctx);
 int64_t wamp3_ctx_scratch_bytes(const wamp3_ctx *ctx);
 /* Interior repair + a1 clamp, as wamp_prepare. */

@@ -95,7 +95,7 @@ def test_ctx3_create_rejects_bad_grids_and_schemes(kw, scheme, msg):
     ctx = C.c_void_p()
     eos = _lib.WampEos(4.4, 6000.0, 1.4, 0.0)
     sch = _lib.WampScheme(*scheme, 1, 0.4)
-    rc = lib.wamp3_ctx_create(C.byref(_grid3(**kw)), C.byref(eos), C.byref(sch), C.byref(ctx))
+    rc = lib.wamp3_ctx_create(C.byref(_grid3(**kw)), C.byref(eos), C.byref(sch), 0, C.byref(ctx))
     assert rc < 0 and not ctx.value
     assert msg in lib.wamp_last_error().decode()
 
@@ -108,3 +108,12 @@ def test_3d_entry_points_reject_null_contexts():
     assert lib.wamp3_prepare(None, U, None) < 0
     assert lib.wamp3_ctx_scratch_bytes(None) == -1
     assert lib.wamp3_ctx_destroy(None) == 0
+
+
+def test_ctx3_create_rejects_an_unknown_precision():
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    eos = _lib.WampEos(4.4, 6000.0, 1.4, 0.0)
+    sch = _lib.WampScheme(0, 0, 1, 1, 1, 0.4)
+    rc = lib.wamp3_ctx_create(C.byref(_grid3()), C.byref(eos), C.byref(sch), 2, C.byref(ctx))
+    assert rc < 0 and not ctx.value and "precision" in lib.wamp_last_error().decode()

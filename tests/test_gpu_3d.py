@@ -135,3 +135,87 @@ def test_scratch_is_68_planes(gpu_ok):
     g = cfg.grid
     assert s.scratch_bytes() == (68 * g.nxg * g.nyg * g.nzg + 32) * 8
     s.close()
+
+
+def test_mixed_precision_stays_close_to_fp64(gpu_ok):
+    """FP32-mixed 3D faces (energy flux, pressure, indicators, limiters FP64)
+    against the FP64 3D step on the shock sphere, same dt sequence: within the
+    1e-4 relative L1 of the 2D mixed path, and really different arithmetic."""
+    import dataclasses
+
+    from wamp import threed
+
+    cfg, U0 = threed.shock_sphere(24)
+    runs = []
+    for prec in (0, 1):
+        s = threed.Solver3(dataclasses.replace(cfg, precision=prec))
+        s.set_state(U0)
+        s.prepare()
+        runs.append(s)
+    runs[0].compute_dt()
+    for _ in range(40):
+        dt = float(runs[0].dt_buf[0].item())
+        runs[0].step_device()
+        runs[1].step_with(dt)
+    ref, mixed = runs[0].state(), runs[1].state()
+    for s in runs:
+        s.close()
+    err = threed.compare3(cfg, mixed, ref)
+    print("3D mixed vs fp64 after 40 steps:", err)
+    assert err["finite"]
+    for k in ("rho", "p", "alpha1"):
+        assert err[k] <= 1e-4, (k, err)
+    assert 0.0 < max(err["rho"], err["p"])
+
+
+@pytest.mark.parametrize("embed", ["xy", "xz"])
+def test_mixed_reduction_is_close_to_the_2d_mixed_step(gpu_ok, embed):
+    """Mixed 3D against mixed 2D on the embedded case.  Not bitwise: the 2D
+    mixed face kernel recomputes pressure and entropy in the mixed unit (one-
+    ulp division, exp(g log rho) entropy), while the 3D mixed faces read the
+    FP64 unit's pressure and entropy planes; a one-ulp FP64 difference can
+    flip an FP32 rounding.  Measured 1.7e-8 after 10 steps on one MI355X; the
+    bound is 1e-6, a hundredth of the mixed path's 1e-4 tolerance."""
+    import dataclasses
+
+    from wamp import cases, threed
+    from wamp.config import PREC_MIXED, SCHEMES
+    from wamp.solver import Solver
+
+    cfg2, U2 = cases.shock_cylinder(64, 32, scheme=SCHEMES["m10"])
+    cfg2 = dataclasses.replace(cfg2, precision=PREC_MIXED)
+    cfg3, U3 = (threed.from_2d_xy if embed == "xy" else threed.from_2d_xz)(cfg2, U2)
+    cfg3 = dataclasses.replace(cfg3, precision=PREC_MIXED)
+    s2 = Solver(cfg2)
+    s2.set_state(U2)
+    s2.prepare()
+    s3 = threed.Solver3(cfg3)
+    s3.set_state(U3)
+    s3.prepare()
+    s2.compute_dt()
+    for _ in range(10):
+        dt = float(s2.dt_buf[0].item())
+        s2.step_device()
+        s3.step_with(dt)
+    out2, out3 = s2.state(), s3.state()
+    s2.close()
+    s3.close()
+    worst = 0.0
+    for name, a, b in _slices(cfg2, out2, cfg3, out3, embed):
+        den = np.abs(b).sum()
+        worst = max(worst, float(np.abs(a - b).sum() / den) if den > 0 else float(np.abs(a).max()))
+    print(f"mixed {embed} reduction: worst relative L1 {worst:.3e}")
+    assert worst < 1e-6
+
+
+def test_mixed_scratch_holds_fp32_face_planes(gpu_ok):
+    import dataclasses
+
+    from wamp import threed
+
+    cfg, U0 = threed.shock_sphere(8)
+    s = threed.Solver3(dataclasses.replace(cfg, precision=1))
+    g = cfg.grid
+    n = g.nxg * g.nyg * g.nzg
+    assert s.scratch_bytes() == (6 + 20) * n * 8 + 42 * n * 4 + 32 * 8
+    s.close()

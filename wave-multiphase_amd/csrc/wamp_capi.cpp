@@ -67,6 +67,9 @@ hipError_t wamp3_launch_cells(const Geo3*, const Eos*, const State3*, double* P,
 hipError_t wamp3_launch_faces(const Geo3*, const Eos*, const State3*, const double* P,
                               const double* S, const double* D, const Faces3*, int base, int shear,
                               int vf_always, int face_pp, double t1, hipStream_t);
+hipError_t wamp3_launch_faces_mixed(const Geo3*, const Eos*, const State3*, const double* P,
+                                    const double* S, const double* D, const Faces3*, int base,
+                                    int shear, int vf_always, int face_pp, double t1, hipStream_t);
 hipError_t wamp3_launch_limit_update(const Geo3*, const Eos*, const State3* Un, const State3* U,
                                      const Faces3*, double* const th[3], const State3* Uo,
                                      double alpha, double beta, const double* dt_dev,
@@ -852,6 +855,7 @@ struct wamp3_ctx {
   Geo3 geo;
   Eos eos;
   wamp_scheme scheme;
+  int precision;
   double t1;
   double* block;
   int64_t bytes;
@@ -872,7 +876,7 @@ static int state3_of(double* const U[7], State3& s, const char* name) {
 }
 
 int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_scheme* scheme,
-                     wamp3_ctx** out) {
+                     int32_t precision, wamp3_ctx** out) {
   if (!out) return fail(-1, "out is NULL");
   *out = nullptr;
   if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
@@ -894,6 +898,8 @@ int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_sch
     return fail(-4, "3D: scheme (base %d, shear %d, face_pp %d) is not built; available: SC (0,1,0), "
                 "M10 (0,0,1), MP5 (1,2,0)", scheme->base, scheme->shear, scheme->face_pp);
   if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
+  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
+    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
 
   wamp3_ctx* c = new (std::nothrow) wamp3_ctx();
   if (!c) return fail(-5, "host allocation failed");
@@ -910,15 +916,23 @@ int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_sch
   for (int k = 0; k < 6; ++k) G.bc[k] = grid->bc[k];
   c->eos = make_eos(*eos);
   c->scheme = *scheme;
+  c->precision = precision;
   c->t1 = std::tanh(1.8 * 0.5);
   const size_t plane = (size_t)plane64;
-  const size_t bytes = ((48 + 14 + 3 + 3) * plane + 32) * sizeof(double);
+  // face planes: 48 FP64, or in mixed precision 6 FP64 energy planes and 42
+  // FP32 planes (a multiple of 8 bytes: 42 * 4 * plane)
+  const size_t fbytes = precision == 0 ? 48 * plane * sizeof(double)
+                                       : 6 * plane * sizeof(double) + 42 * plane * sizeof(float);
+  const size_t bytes = fbytes + ((14 + 3 + 3) * plane + 32) * sizeof(double);
   int rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(3D scratch)");
   if (rc) { delete c; return rc; }
   c->bytes = (int64_t)bytes;
-  double* p = c->block;
-  c->faces = Faces3{p, (long long)plane};
-  p += 48 * plane;
+  c->faces = Faces3{c->block, (long long)plane};
+  if (precision != 0) {
+    c->faces.h = (float*)((char*)c->block + 6 * plane * sizeof(double));
+    c->faces.hs = (long long)plane;
+  }
+  double* p = (double*)((char*)c->block + fbytes);
   for (int m = 0; m < 7; ++m, p += plane) c->u1.f[m] = p;
   for (int m = 0; m < 7; ++m, p += plane) c->u2.f[m] = p;
   c->P = p; p += plane;
@@ -980,8 +994,9 @@ int wamp3_step(wamp3_ctx* c, double* const U[7], const double* dt_dev, double* d
     if (int rc = hip_check(wamp3_launch_cells(&c->geo, &c->eos, cur[s], c->P, c->S, c->D, sc.shear, st),
                            "3D cells"))
       return rc;
-    if (int rc = hip_check(wamp3_launch_faces(&c->geo, &c->eos, cur[s], c->P, c->S, c->D, &c->faces,
-                                              sc.base, sc.shear, sc.vf_thinc, sc.face_pp, c->t1, st),
+    auto faces = c->precision == 0 ? wamp3_launch_faces : wamp3_launch_faces_mixed;
+    if (int rc = hip_check(faces(&c->geo, &c->eos, cur[s], c->P, c->S, c->D, &c->faces, sc.base,
+                                 sc.shear, sc.vf_thinc, sc.face_pp, c->t1, st),
                            "3D faces"))
       return rc;
     if (int rc = hip_check(wamp3_launch_limit_update(&c->geo, &c->eos, &a, cur[s], &c->faces, c->th,

@@ -84,13 +84,23 @@ struct State3 {  // [a1 rho1, a2 rho2, rho u, rho v, rho w, E, a1]
 };
 // The 48 face planes of a 3D step: direction d (x, y, z) holds 16 planes at
 // d * 16: the high-order fluxes of the 7 conserved variables and u*, then
-// the first-order ones; plane m starts at b + m * s.  The face d + 1/2 of
-// cell (i, j, k) is stored at the cell's index.
+// the first-order ones.  The face d + 1/2 of cell (i, j, k) is stored at
+// the cell's index.
+//  * FP64 storage (h == nullptr): plane m starts at b + m * s.
+//  * Mixed storage (the FP32-mixed face kernels, whose fluxes other than the
+//    energy flux are FP32 values): the six energy planes (m % 8 == 5) in
+//    FP64 at b + face3_slot64(m) * s, the other 42 in FP32 at
+//    h + face3_slot32(m) * hs.
 struct Faces3 {
   double* b;
   long long s;
-  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }
+  float* h = nullptr;
+  long long hs = 0;
+  __host__ __device__ double* operator[](int m) const { return b + (long long)m * s; }  // FP64 storage
 };
+__host__ __device__ constexpr bool face3_is_energy(int m) { return m % 8 == 5; }
+__host__ __device__ constexpr int face3_slot64(int m) { return m / 8; }
+__host__ __device__ constexpr int face3_slot32(int m) { return (m / 8) * 7 + (m % 8 < 5 ? m % 8 : m % 8 - 1); }
 
 // Per-slot device scalars of a batch.
 struct BatchScal {

@@ -30,6 +30,7 @@ UNITS = [
     ("wamp_kernels.hip", "-ffp-contract=off"),
     ("wamp_faces_mixed.hip", "-ffp-contract=on"),
     ("wamp3_kernels.hip", "-ffp-contract=off"),
+    ("wamp3_faces_mixed.hip", "-ffp-contract=on"),
     ("wamp_capi.cpp", "-ffp-contract=off"),
 ]
 

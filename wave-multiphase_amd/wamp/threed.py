@@ -83,6 +83,7 @@ class Case3Config:
     eos: EOS = field(default_factory=lambda: WATER_ND)
     scheme: Scheme = SCHEME_M10
     name: str = "case3"
+    precision: int = 0  # 0 FP64, 1 FP32-mixed face arithmetic
 
 
 def conserved3(eos: EOS, a1, rho1, rho2, u, v, w, p):
@@ -145,6 +146,29 @@ def from_2d_xz(cfg2: CaseConfig, U2: np.ndarray, ny: int = 4, bc_y=(BC_SLIP_WALL
     return Case3Config(grid, cfg2.eos, cfg2.scheme, name=cfg2.name + "_xz"), U
 
 
+def pressure3(eos: EOS, U):
+    """Pressure of a (7, ...) state (cons_to_prim SC:856-860 with w), numpy."""
+    al = U[6]
+    rho = U[0] + U[1]
+    u, v, w = U[2] / rho, U[3] / rho, U[4] / rho
+    a2 = 1.0 - al
+    gb = al / (eos.gamma1 - 1.0) + a2 / (eos.gamma2 - 1.0)
+    pb = al * eos.gamma1 * eos.pinf1 / (eos.gamma1 - 1.0) + a2 * eos.gamma2 * eos.pinf2 / (eos.gamma2 - 1.0)
+    return (U[5] - pb - 0.5 * rho * ((u * u + v * v) + w * w)) / gb
+
+
+def compare3(cfg: Case3Config, U, Uref) -> dict:
+    """Relative L1 errors of density, pressure and alpha1 over the interior."""
+    from .accuracy import rel_l1
+
+    si, sj, sk = cfg.grid.interior()
+    I, R = U[:, si, sj, sk], Uref[:, si, sj, sk]
+    return {"rho": rel_l1(I[0] + I[1], R[0] + R[1]),
+            "p": rel_l1(pressure3(cfg.eos, I), pressure3(cfg.eos, R)),
+            "alpha1": rel_l1(I[6], R[6]),
+            "finite": bool(np.isfinite(I).all())}
+
+
 def _torch():
     import torch
 
@@ -175,8 +199,8 @@ class Solver3:
         s = cfg.scheme
         sch = _lib.WampScheme(s.base, s.shear, s.vf_thinc, s.face_pp, 1 if s.repair else 0, s.cfl)
         ctx = C.c_void_p()
-        _lib.check(self.L.wamp3_ctx_create(C.byref(grid), C.byref(eos), C.byref(sch), C.byref(ctx)),
-                   "wamp3_ctx_create")
+        _lib.check(self.L.wamp3_ctx_create(C.byref(grid), C.byref(eos), C.byref(sch),
+                                           int(cfg.precision), C.byref(ctx)), "wamp3_ctx_create")
         self.ctx = ctx
         self.nstep = 0
 
@@ -235,6 +259,13 @@ class Solver3:
     def step_with(self, dt: float):
         """One step with the given dt (the CFL dt of the result is discarded)."""
         self.set_dt(dt)
+        _lib.check(self.L.wamp3_step(self.ctx, self._u, self.dt_ptr, self.dt_buf[1:].data_ptr(),
+                                     self.stream), "wamp3_step")
+        self.nstep += 1
+
+    def step_with_device_dt(self):
+        """One step with the dt already in the device slot, which it keeps (the
+        CFL dt of the result goes to the second slot)."""
         _lib.check(self.L.wamp3_step(self.ctx, self._u, self.dt_ptr, self.dt_buf[1:].data_ptr(),
                                      self.stream), "wamp3_step")
         self.nstep += 1
