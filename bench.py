@@ -389,7 +389,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--ny", type=int, default=2048)
-    ap.add_argument("--cpu-steps", type=int, default=2,
+    ap.add_argument("--cpu-steps", type=int, default=5,
                     help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the FP32-mixed leg on the same case (0 = skip)")
