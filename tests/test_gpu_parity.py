@@ -97,6 +97,29 @@ def test_200_steps_match_oracle(gpu_ok, scheme):
     assert max(errs) <= 1e-9
 
 
+@pytest.mark.parametrize("nx, ny", [(1, 1), (1, 6), (6, 1), (3, 2), (37, 23), (130, 3)])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_degenerate_and_ragged_grids_match_oracle(gpu_ok, scheme, nx, ny):
+    """One cell, single rows and columns, and sizes that are no multiple of any
+    tile or strip: 20 steps against the oracle (every ghost cell maps to an
+    edge cell; most tiles are partly outside the grid)."""
+    from wamp.solver import Solver
+
+    cfg, orc = _setup(scheme, nx=nx, ny=ny)
+    U0 = orc.state()
+    dts_ref = orc.run(20)
+    ref = orc.state()
+    s = Solver(cfg)
+    s.set_state(U0)
+    dts = s.run(20)
+    got = s.state()
+    s.close()
+    errs = _interior_rel(cfg, got, ref)
+    assert np.isfinite(got).all()
+    assert max(errs) <= 1e-9, errs
+    assert np.abs(dts - dts_ref).max() <= 1e-12 * dts_ref.max()
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("scheme", ["m10", "mp5"])
 def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, precision, monkeypatch):
