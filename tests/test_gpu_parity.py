@@ -149,3 +149,32 @@ def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, precision, monkeypatch):
     for k, (st, dn) in outs.items():
         assert np.array_equal(st[:, si, sj], ref[:, si, sj]), k
         assert dn == dref, k
+
+
+@pytest.mark.parametrize("nx, ny", [(1, 1), (6, 1), (3, 2), (37, 23), (130, 3), (9, 61)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_march_limiter_on_degenerate_grids(gpu_ok, nx, ny, precision, monkeypatch):
+    """The marching limiter (chosen only above 2^18 cells) forced onto grids of
+    one cell, one row, and widths that end mid-strip or span two strips: 10
+    steps bitwise equal to the LDS-tile limiter, in FP64 and mixed storage."""
+    import dataclasses
+
+    from wamp.solver import Solver
+
+    cfg, orc = _setup("m10", nx=nx, ny=ny)
+    cfg = dataclasses.replace(cfg, precision=precision)
+    U = orc.state()
+    outs = {}
+    for kind, seg in (("0", "0"), ("1", "4"), ("1", "26")):
+        monkeypatch.setenv("WAMP_LU_KIND", kind)
+        monkeypatch.setenv("WAMP_LU_SEG", seg)
+        s = Solver(cfg)
+        s.set_state(U)
+        s.run(10)
+        outs[(kind, seg)] = s.state()
+        s.close()
+    si, sj = cfg.grid.interior()
+    ref = outs[("0", "0")]
+    assert np.isfinite(ref[:, si, sj]).all()
+    for k, st in outs.items():
+        assert np.array_equal(st[:, si, sj], ref[:, si, sj]), k
