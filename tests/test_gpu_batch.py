@@ -103,3 +103,32 @@ def test_autotune_keeps_results(gpu_ok):
         si, sj = c.grid.interior()
         assert np.array_equal(b.state(k)[:, si, sj], ref[k][:, si, sj])
     b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_with_degenerate_cases_equals_alone(gpu_ok, precision):
+    """One-cell, one-row and ragged cases beside a normal one in the same
+    persistent launches: each bitwise equal to its solve alone."""
+    from wamp import cases
+    from wamp.batch import Batch, Tuning
+    from wamp.config import SCHEME_M10
+
+    shapes = [(1, 1), (6, 1), (3, 2), (130, 3), (9, 61), (96, 48)]
+    cs = [(dataclasses.replace(c, precision=precision), U)
+          for c, U in (cases.shock_cylinder(nx, ny, scheme=SCHEME_M10) for nx, ny in shapes)]
+    steps = 5
+    ref = [_alone(c, U, steps) for c, U in cs]
+    for tuning in (Tuning(0, 3, 2, 8), Tuning(1, 2, 2, 26)):
+        b = Batch(SCHEME_M10, precision, max_cases=len(cs))
+        for k, (c, U) in enumerate(cs):
+            b.set_case(k, c, U)
+        b.set_tuning(tuning)
+        b.commit()
+        b.prepare()
+        done = 0
+        while done < steps:
+            done += b.step()
+        for k, (c, _) in enumerate(cs):
+            si, sj = c.grid.interior()
+            assert np.array_equal(b.state(k)[:, si, sj], ref[k][:, si, sj]), (tuning, shapes[k])
+        b.close()
