@@ -101,3 +101,39 @@ def test_full_size_many_steps_match_oracle(gpu_ok, case):
     print(f"4096x2048, {n} steps:", ["%.2e" % e for e in errs],
           "dt rel", float(np.abs(dts - dts_ref).max() / dts_ref.max()))
     assert max(errs) <= 1e-9
+
+
+@pytest.mark.skipif(os.environ.get("WAMP_MAXSIZE") != "1",
+                    reason="opt-in (WAMP_MAXSIZE=1): 16384x8192, ~80 GB of host memory, ~1 minute")
+def test_largest_grid_step_matches_oracle(gpu_ok):
+    """16384 x 8192 (134 M cells, 16x the headline): one step against the oracle
+    -- 64-bit offsets, launch grids and the march's strip / segment counts at
+    the size where 32-bit cell indices would no longer be safe for ld * nx."""
+    from oracle.oracle import Oracle
+    from wamp import cases
+    from wamp.config import SCHEME_M10
+    from wamp.solver import Solver
+
+    cfg, U0 = cases.mach10_cylinder(16384, 8192, scheme=SCHEME_M10)
+    orc = Oracle(cfg, threads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    orc.set_state(U0)
+    del U0
+    orc.prepare()
+    U = orc.state()
+    dt = orc.get_dt()
+    orc.step(dt)
+    ref = orc.state()
+    dt_ref = orc.get_dt()
+    orc.close()
+    s = Solver(cfg)
+    s.set_state(U)
+    s.set_dt(dt)
+    s.step_device()
+    got = s.state()
+    dt_next = float(s.dt_buf[0].item())
+    s.close()
+    si, sj = cfg.grid.interior()
+    errs = [rel_l1(got[m][si, sj], ref[m][si, sj]) for m in range(6)]
+    print("16384x8192 one step:", ["%.2e" % e for e in errs])
+    assert max(errs) <= 1e-12
+    assert abs(dt_next - dt_ref) <= 1e-14 * dt_ref
