@@ -210,3 +210,29 @@ def test_boundary_variants_match_oracle(gpu_ok, scheme, bc):
     errs = _interior_rel(cfg, got, ref)
     assert np.isfinite(got).all()
     assert max(errs) <= 1e-9, (bc, errs)
+
+
+@pytest.mark.parametrize("lu_kind", ["0", "1"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_padded_row_stride_gives_the_same_result(gpu_ok, precision, lu_kind, monkeypatch):
+    """A state whose rows are padded (ld = ny + 2g + 13, not a multiple of
+    anything) steps bitwise like the unpadded one, with either limiter kernel."""
+    import dataclasses
+
+    from wamp import cases
+    from wamp.config import SCHEME_M10
+    from wamp.solver import Solver
+
+    monkeypatch.setenv("WAMP_LU_KIND", lu_kind)
+    cfg, U0 = cases.shock_cylinder(200, 120, scheme=SCHEME_M10)
+    cfg = dataclasses.replace(cfg, precision=precision)
+    outs = []
+    for ld in (None, cfg.grid.nyg + 13):
+        s = Solver(cfg, ld=ld)
+        s.set_state(U0)
+        s.prepare()
+        s.run(20)
+        outs.append(s.state())
+        s.close()
+    si, sj = cfg.grid.interior()
+    assert np.array_equal(outs[0][:, si, sj], outs[1][:, si, sj])
