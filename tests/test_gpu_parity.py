@@ -236,3 +236,35 @@ def test_padded_row_stride_gives_the_same_result(gpu_ok, precision, lu_kind, mon
         s.close()
     si, sj = cfg.grid.interior()
     assert np.array_equal(outs[0][:, si, sj], outs[1][:, si, sj])
+
+
+@pytest.mark.parametrize("eos", [(4.4, 6000.0, 2.0, 100.0), (1.67, 0.0, 1.4, 0.0), (6.12, 3430.0, 1.4, 0.0)])
+@pytest.mark.parametrize("scheme", ["m10", "mp5"])
+def test_eos_variants_match_oracle(gpu_ok, scheme, eos):
+    """Other stiffened-gas pairs: two stiffened phases (both p_inf > 0), two ideal
+    gases, a stiffer liquid.  20 steps of a 128x64 shock-droplet state against
+    the oracle."""
+    from oracle.oracle import Oracle
+    from wamp import cases
+    from wamp.config import EOS, SCHEMES as S, CaseConfig, Grid
+    from wamp.solver import Solver
+
+    e = EOS(*eos)
+    grid = Grid(128, 64, 0.0, 4.0, 0.0, 2.0)
+    cfg = CaseConfig(grid, e, S[scheme], name="eos_variant")
+    U0 = cases.shock_droplets(grid, e, 2.4, 0.8, [(1.5, 1.0, 0.5, 0.2)])
+    orc = Oracle(cfg)
+    orc.set_state(U0)
+    orc.prepare()
+    U = orc.state()
+    orc.run(20)
+    ref = orc.state()
+    orc.close()
+    s = Solver(cfg)
+    s.set_state(U)
+    s.run(20)
+    got = s.state()
+    s.close()
+    errs = _interior_rel(cfg, got, ref)
+    assert np.isfinite(got).all()
+    assert max(errs) <= 1e-9, (eos, errs)
