@@ -293,3 +293,23 @@ def test_generic_3d_state_commutes_with_axis_swaps(gpu_ok, axes, scheme):
         err = np.abs(SA[m] - B[m]).max() / scale
         assert err < 1e-12, (scheme, axes, m, err)
     assert (moved[2:5] > 0).all()  # every momentum changed over the 5 steps
+
+
+def test_cli_writes_outputs_and_restarts(gpu_ok, tmp_path):
+    """python -m wamp.threed: 4 steps saved every 2, then a restart from the
+    step-2 file for 2 more steps reaches the step-4 state. The restart
+    rebuilds E from p, so the match is to round-off, not bitwise."""
+    from wamp import threed
+
+    a, b = tmp_path / "a", tmp_path / "b"
+    assert threed.main(["--n", "8", "--steps", "4", "--save-every", "2", "--out", str(a)]) == 0
+    files = sorted(p.name for p in a.iterdir())
+    assert files == ["shock_sphere_16x8x8_000002.npz", "shock_sphere_16x8x8_000004.npz"], files
+    assert threed.main(["--n", "8", "--steps", "2", "--restart", str(a / files[0]), "--out", str(b)]) == 0
+    assert [p.name for p in b.iterdir()] == [files[1]]
+    with np.load(a / files[1]) as A, np.load(b / files[1]) as B:
+        assert float(A["time"]) == pytest.approx(float(B["time"]), rel=1e-12)
+        assert float(A["time"]) > 0
+        for k in ("rho", "u", "v", "w", "p", "alpha1"):
+            x, y = A[k], B[k]
+            assert np.abs(x - y).sum() <= 1e-9 * np.abs(x).sum() + 1e-12, k

@@ -45,3 +45,31 @@ def test_shock_sphere_state_is_admissible_and_y_z_symmetric():
     # water sphere in the middle, post-shock gas on the left
     c = g.g + g.nx * 3 // 8, g.g + g.ny // 2, g.g + g.nz // 2
     assert U[6][c] > 0.5 and U[2][g.g, c[1], c[2]] > 0.0
+
+
+def test_npz_round_trip(tmp_path):
+    """save_npz3 then load_restart3 gives back the interior state (E rebuilt from
+    p with the same EOS, to round-off) and the time; ghosts stay zero."""
+    cfg, U = threed.shock_sphere(8)
+    path = threed.save_npz3(str(tmp_path / "s_000010.npz"), cfg.grid, U, 0.125, cfg.eos)
+    V, t = threed.load_restart3(path, cfg)
+    assert t == 0.125 and V.shape == U.shape
+    si, sj, sk = cfg.grid.interior()
+    for m in range(7):
+        a, b = V[m][si, sj, sk], U[m][si, sj, sk]
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max()), m
+    assert not V[:, : cfg.grid.g].any()
+    with np.load(path) as d:
+        assert sorted(d.files) == sorted(["time", "x", "y", "z", "rho", "ar1", "ar2", "u", "v", "w",
+                                          "p", "alpha1"])
+        assert d["rho"].shape == (cfg.grid.nx, cfg.grid.ny, cfg.grid.nz)
+
+
+def test_restart_rejects_a_wrong_shape(tmp_path):
+    import pytest
+
+    cfg, U = threed.shock_sphere(8)
+    path = threed.save_npz3(str(tmp_path / "a.npz"), cfg.grid, U, 0.0, cfg.eos)
+    other, _ = threed.shock_sphere(6)
+    with pytest.raises(ValueError):
+        threed.load_restart3(path, other)

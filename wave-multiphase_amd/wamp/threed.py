@@ -169,6 +169,50 @@ def compare3(cfg: Case3Config, U, Uref) -> dict:
             "finite": bool(np.isfinite(I).all())}
 
 
+REQUIRED3 = ["ar1", "ar2", "u", "v", "w", "p", "alpha1"]
+
+
+def save_npz3(path: str, grid: Grid3, U, t: float, eos: EOS) -> str:
+    """Interior of a 3D state in the npz layout of the 2D writer (save_npz,
+    SC:1712-1726) with z and w added: keys time, x, y, z, rho, ar1, ar2, u, v,
+    w, p, alpha1, arrays of shape (nx, ny, nz)."""
+    si, sj, sk = grid.interior()
+    I = np.asarray(U)[:, si, sj, sk]
+    rho = I[0] + I[1]
+    np.savez_compressed(path, time=np.float64(t),
+                        x=grid.centres(0)[si], y=grid.centres(1)[sj], z=grid.centres(2)[sk],
+                        rho=rho, ar1=I[0], ar2=I[1], u=I[2] / rho, v=I[3] / rho, w=I[4] / rho,
+                        p=pressure3(eos, I), alpha1=I[6])
+    return path
+
+
+def load_restart3(path: str, cfg: Case3Config):
+    """(U (7, NXG, NYG, NZG) with zero ghosts, time) from a save_npz3 file, as
+    load_restart (SC:1779-1821): required keys checked, alpha1 clipped to
+    [1e-8, 1 - 1e-8], E rebuilt from p with the run's EOS."""
+    g, e = cfg.grid, cfg.eos
+    with np.load(path, allow_pickle=False) as data:
+        for key in REQUIRED3:
+            if key not in data:
+                raise KeyError(f"Restart file {path} is missing '{key}'")
+        f = {k: np.asarray(data[k], dtype=np.float64) for k in REQUIRED3}
+        t = float(data["time"]) if "time" in data else 0.0
+    if f["ar1"].shape != (g.nx, g.ny, g.nz):
+        raise ValueError(f"Restart shape {f['ar1'].shape} does not match {(g.nx, g.ny, g.nz)}")
+    al = np.clip(f["alpha1"], 1.0e-8, 1.0 - 1.0e-8)
+    rho = f["ar1"] + f["ar2"]
+    a2 = 1.0 - al
+    gb = al / (e.gamma1 - 1.0) + a2 / (e.gamma2 - 1.0)
+    pb = al * e.gamma1 * e.pinf1 / (e.gamma1 - 1.0) + a2 * e.gamma2 * e.pinf2 / (e.gamma2 - 1.0)
+    u, v, w = f["u"], f["v"], f["w"]
+    E = gb * f["p"] + pb + 0.5 * rho * ((u * u + v * v) + w * w)
+    U = np.zeros((7, g.nxg, g.nyg, g.nzg), dtype=np.float64)
+    si, sj, sk = g.interior()
+    for m, a in enumerate((f["ar1"], f["ar2"], rho * u, rho * v, rho * w, E, al)):
+        U[m][si, sj, sk] = a
+    return U, t
+
+
 def _torch():
     import torch
 
@@ -279,3 +323,56 @@ class Solver3:
             dts[k].copy_(self.dt_buf[0])
             self.step_device()
         return dts.cpu().numpy()
+
+
+def main(argv=None) -> int:
+    """python -m wamp.threed: run the 3D shock-sphere case and write npz outputs."""
+    import argparse
+    import os
+    import sys
+    import time
+
+    from .config import SCHEMES
+    from .io import infer_restart_step, output_name
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--n", type=int, default=64, help="grid 2n x n x n")
+    ap.add_argument("--scheme", choices=sorted(SCHEMES), default="m10")
+    ap.add_argument("--precision", type=int, choices=(0, 1), default=0)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--save-every", type=int, default=0, help="also write every K steps (0: last only)")
+    ap.add_argument("--out", default="out3d")
+    ap.add_argument("--restart", default=None, help="npz written by this program")
+    args = ap.parse_args(argv)
+    cfg, U0 = shock_sphere(args.n, scheme=SCHEMES[args.scheme])
+    cfg = Case3Config(cfg.grid, cfg.eos, cfg.scheme, cfg.name, args.precision)
+    t, step0 = 0.0, 0
+    if args.restart:
+        U0, t = load_restart3(args.restart, cfg)
+        step0 = infer_restart_step(args.restart)
+    os.makedirs(args.out, exist_ok=True)
+    s = Solver3(cfg)
+    s.set_state(U0)
+    s.prepare()
+    prefix = os.path.join(args.out, cfg.name)
+    t0 = time.perf_counter()
+    dts = []
+    done = 0
+    while done < args.steps:
+        n = min(args.steps - done, args.save_every or args.steps)
+        dts.extend(s.run(n))
+        done += n
+        t_now = t + float(np.sum(dts))
+        path = save_npz3(output_name(prefix, step0 + done), cfg.grid, s.state(), t_now, cfg.eos)
+        print(f"step {step0 + done}  t {t_now:.6g}  wrote {path}", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    g = cfg.grid
+    print(f"{args.steps} steps of {g.nx}x{g.ny}x{g.nz} in {el:.2f} s", file=sys.stderr, flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(main())
