@@ -340,6 +340,43 @@ def measured_traffic(workload: str, kernel: str):
     return int(vals["traffic_bytes"])
 
 
+def copy_rate_gbs(mib: int = 1024, reps: int = 5) -> float:
+    """Device-to-device copy rate of a MIB-MiB buffer (read + write bytes per
+    second, GB/s), timed with events over REPS copies after one warm-up: the
+    HBM rate a plain streaming kernel reaches on this device."""
+    import torch
+
+    a = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * a.numel() * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    return gbs
+
+
+def limiter_rates(workload: str, launch_ms: float, copy_gbs: float) -> dict:
+    """The limiter launch's HBM rates: algorithmic bytes and the committed
+    rocprofv3 traffic per launch (profiles/), each over the live launch time,
+    beside the published peak and the measured copy rate."""
+    vals = _record(f"k_lu_march_traffic_{workload}.txt")
+    out = {"kernel": "k_lu_march", "launch_ms": round(launch_ms, 4)}
+    if vals.get("workload") != workload or vals.get("kernel") != "k_lu_march":
+        return dict(out, traffic=None)
+    t = launch_ms / 1e3
+    alg, traffic = int(vals["alg_bytes"]), int(vals["traffic_bytes"])
+    return dict(out, alg_bytes=alg, traffic=traffic,
+                alg_gbs=round(alg / t / 1e9, 1), traffic_gbs=round(traffic / t / 1e9, 1),
+                traffic_frac_of_peak=round(traffic / t / 1e9 / HBM_PEAK_GBS, 4),
+                traffic_frac_of_copy=round(traffic / t / 1e9 / copy_gbs, 4))
+
+
 def measured_step_work(workload: str):
     """(FP64 FLOP, VALU wave instructions) of one FP64 step of `workload` from
     the committed rocprofv3 instruction counts, or None."""
@@ -431,6 +468,7 @@ def main(argv=None):
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    copy_gbs = copy_rate_gbs()
     cfg, U0 = cases.mach10_cylinder(args.nx, args.ny, scheme=SCHEME_M10)
     g = cfg.grid
     s = Solver(cfg, device="cuda:0")
@@ -504,6 +542,8 @@ def main(argv=None):
             **kernel_valu(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces", faces_ms),
         },
         "kernels_ms": {"k_faces": round(faces_ms, 4), "k_limit_update": round(lu_ms, 4)},
+        "limiter": limiter_rates(f"mach10_cylinder_{g.nx}x{g.ny}", lu_ms, copy_gbs),
+        "hbm_copy_gbs": round(copy_gbs, 1),
         "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 5),
         "step_valu_frac": valu_fracs(f"mach10_cylinder_{g.nx}x{g.ny}", t_ms / K),
         "state_finite": finite,

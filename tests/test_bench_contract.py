@@ -27,3 +27,15 @@ def test_traffic_record_lookup(tmp_path, monkeypatch):
     (prof / "k_faces_traffic_x_1x1.txt").write_text("workload=other\nkernel=k_faces\ntraffic_bytes=5\n")
     monkeypatch.setattr(bench, "ROOT", str(tmp_path))
     assert bench.measured_traffic("x_1x1", "k_faces") is None
+
+
+def test_limiter_rates_from_the_committed_record():
+    sys.path.insert(0, ROOT)
+    import bench
+
+    r = bench.limiter_rates("mach10_cylinder_4096x2048", 0.861, 6000.0)
+    assert r["alg_bytes"] == 368 * 4096 * 2048
+    assert r["traffic"] > r["alg_bytes"]
+    assert abs(r["traffic_gbs"] - r["traffic"] / 0.861e-3 / 1e9) < 0.1
+    assert abs(r["traffic_frac_of_copy"] - r["traffic_gbs"] / 6000.0) < 1e-4
+    assert bench.limiter_rates("mach10_cylinder_512x256", 0.1, 6000.0)["traffic"] is None
