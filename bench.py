@@ -358,6 +358,7 @@ def copy_rate_gbs(mib: int = 1024, reps: int = 5) -> float:
     torch.cuda.synchronize()
     gbs = 2.0 * a.numel() * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
     del a, b
+    torch.cuda.empty_cache()
     return gbs
 
 
