@@ -13,6 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
+# the same source built with every double counted by operation kind (opcount.h)
+COUNT_LIB_PATH = os.path.join(HERE, "liborc_count.so")
+OPCOUNT_KINDS = ("add", "mul", "div", "sqrt", "pow", "tanh", "fabs", "floor", "cmp")
 
 # array ids, mirror of the enum in wamp_oracle.c
 AR1, AR2, RU, RV, E, AL, UP, VP, PP, ALPRIM = range(10)
@@ -43,23 +46,25 @@ class S6(C.Structure):
         return [self.v0, self.v1, self.v2, self.v3, self.v4, self.v5]
 
 
-def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (gcc, -O2 -ffp-contract=off)."""
-    if force and os.path.exists(LIB_PATH):
-        os.remove(LIB_PATH)
-    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
-    return LIB_PATH
+def build(force: bool = False, counting: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -O2 -ffp-contract=off); with
+    counting, the operation-counting build (g++, same flags, serial)."""
+    path = COUNT_LIB_PATH if counting else LIB_PATH
+    if force and os.path.exists(path):
+        os.remove(path)
+    subprocess.run(["make", "-s", "-C", HERE, os.path.basename(path)], check=True)
+    return path
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = C.CDLL(LIB_PATH)
+def lib(counting: bool = False):
+    path = COUNT_LIB_PATH if counting else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            build(counting=counting)
+        L = C.CDLL(path)
         d, pd, ip = C.c_double, C.POINTER(C.c_double), C.c_void_p
         pe = C.POINTER(OrcEos)
         L.orc_create.restype = ip
@@ -106,8 +111,13 @@ def lib():
             f = getattr(L, n)
             f.restype = S6
             f.argtypes = args
-        _lib = L
-    return _lib
+        if counting:
+            L.orc_opcount_read.argtypes = [C.POINTER(C.c_uint64)]
+            L.orc_opcount_categories.restype = C.c_int
+            if L.orc_opcount_categories() != len(OPCOUNT_KINDS):
+                raise RuntimeError("liborc_count.so counts a different set of operation kinds")
+        _libs[path] = L
+    return _libs[path]
 
 
 def eos_struct(eos) -> OrcEos:
@@ -123,7 +133,7 @@ def dvec(values) -> "C.Array":
 class Oracle:
     """One oracle solver context over the reference's 47 padded arrays."""
 
-    def __init__(self, cfg, threads: int | None = None):
+    def __init__(self, cfg, threads: int | None = None, counting: bool = False):
         g = cfg.grid
         s = cfg.scheme
         p = OrcParams()
@@ -137,7 +147,8 @@ class Oracle:
         self.params = p
         self.cfg = cfg
         self.shape = (g.nxg, g.nyg)
-        self._L = lib()
+        self._L = lib(counting)
+        self.counting = counting
         if threads is not None:
             self._L.orc_set_threads(int(threads))
         self._ctx = self._L.orc_create(C.byref(p))
@@ -207,6 +218,17 @@ class Oracle:
 
     def copy_old(self):
         self._L.orc_copy_old(self._ctx)
+
+    def opcounts(self, reset: bool = False) -> dict:
+        """Floating-point operations of each kind since the last reset (the
+        counters are per library, not per context); counting builds only."""
+        if not self.counting:
+            raise RuntimeError("opcounts() needs Oracle(..., counting=True)")
+        out = (C.c_uint64 * len(OPCOUNT_KINDS))()
+        self._L.orc_opcount_read(out)
+        if reset:
+            self._L.orc_opcount_reset()
+        return dict(zip(OPCOUNT_KINDS, (int(v) for v in out)))
 
 
 def max_threads() -> int:

@@ -1,6 +1,8 @@
 /*
  * wamp_oracle.h -- CPU ORACLE (test infrastructure only; see wamp_oracle.c).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ * The same source also builds as liborc_count.so (opcount.h), which counts the
+ * floating-point operations of a step by kind.
  */
 #ifndef WAMP_ORACLE_H
 #define WAMP_ORACLE_H
