@@ -674,10 +674,21 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
            const double* __restrict__ dt_dev, double* dt_acc, int repair) {
   __shared__ double carry[LU_WAVES * LU_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
-  const int seg = (int)blockIdx.y * (blockDim.x / MW) + wave;
+  // Workgroups go to the 8 XCDs round-robin by linear id.  Give each XCD a
+  // contiguous run of (strip, segment group) in strip-fastest order, so that
+  // neighbouring strips, which share their halo columns' cache lines, and
+  // neighbouring segment groups, which share warm-up rows, run on the same
+  // XCD's L2.  The map is a bijection on [0, N); which block does which strip
+  // and segment does not change any result.
+  const int nblk = (int)(gridDim.x * gridDim.y);
+  const int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  const int xcd = lin & 7, base = nblk >> 3, extra = nblk & 7;
+  const int q = xcd * base + (xcd < extra ? xcd : extra) + (lin >> 3);
+  const int strip = q % (int)gridDim.x;
+  const int seg = (q / (int)gridDim.x) * (blockDim.x / MW) + wave;
   if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
   double v = lu_wave<SEG, F32>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
-                          (int)blockIdx.x, seg, carry + wave * (LU_CARRY * MW));
+                          strip, seg, carry + wave * (LU_CARRY * MW));
   if (dt_acc) wave_min_atomic(v, dt_acc);
 }
 
