@@ -39,6 +39,10 @@ class OrcParams(C.Structure):
                 ("cfl", C.c_double)]
 
 
+# orc_step's test hook: fn(user, stage) fills the face fluxes of the stage
+FLUX_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+
+
 class S6(C.Structure):
     _fields_ = [("v%d" % k, C.c_double) for k in range(6)]
 
@@ -83,6 +87,7 @@ def lib(counting: bool = False):
         L.orc_get_dt.argtypes = [ip]
         L.orc_step.argtypes = [ip, d]
         L.orc_run.argtypes = [ip, C.c_int, pd]
+        L.orc_set_flux_hook.argtypes = [ip, FLUX_HOOK, ip]
         # scalar helpers
         L.orc_mix_gamma_pibig.argtypes = [pe, d, pd, pd]
         L.orc_mix_gamma_pinf.argtypes = [pe, d, pd, pd]
@@ -107,6 +112,7 @@ def lib(counting: bool = False):
                         ("orc_to_phys_x", [S6, d, d, d, d, d, d]),
                         ("orc_to_phys_y", [S6, d, d, d, d, d, d]),
                         ("orc_hllc_x", [pe, pd, pd]), ("orc_hllc_y", [pe, pd, pd]),
+                        ("orc_pp_floor_pressure_state", [pe, S6]),
                         ("orc_pp_limit_reconstructed_state", [pe, S6, S6])):
             f = getattr(L, n)
             f.restype = S6
@@ -218,6 +224,13 @@ class Oracle:
 
     def copy_old(self):
         self._L.orc_copy_old(self._ctx)
+
+    def set_flux_hook(self, fn):
+        """While set, step() calls fn(stage) for each RK stage instead of the
+        flux kernels (fn writes the 28 face planes through array()); None
+        restores the kernels."""
+        self._hook = FLUX_HOOK(lambda _user, stage: fn(int(stage))) if fn else None
+        self._L.orc_set_flux_hook(self._ctx, self._hook if fn else FLUX_HOOK(), None)
 
     def opcounts(self, reset: bool = False) -> dict:
         """Floating-point operations of each kind since the last reset (the

@@ -26,21 +26,34 @@
  *   face_pp  0 = simple first-order fallback (SC:1102-1115)
  *            1 = Wong staged face limiter + pressure floor (M10:1007-1025)
  *
- * Parity pins (tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle_pins.py, tests/test_oracle_step.py,
+ * tests/test_oracle_closures.py):
  *   - closed forms for the EOS, sound speed, energy, pressure recovery
- *   - L.R = I for the characteristic projections, x<->y transpose symmetry
+ *   - L.R = I for the characteristic projections, R's columns eigenvectors of
+ *     the flux Jacobian, y projection = rotated x projection
  *   - polynomial exactness of MUSCL (linear), central4 (cubic), central6
  *     (quintic) and MP5 (quartic, limiter inactive); TVD bounds; mirror
  *     symmetry of the left/right reconstructions
  *   - THINC bounds and odd symmetry; indicator on smooth vs step data
  *   - HLLC consistency F(U,U) = physical flux, supersonic upwinding,
- *     stationary contact; Toro-type 1D Riemann problems (converged level)
- *   - Wong limiter no-op on admissible data, forced-failure repair
- *   - SSP-RK3 on y' = lambda*y, free-stream preservation, y-mirror symmetry,
- *     conservation with reflective walls
+ *     stationary contact; exact 1D Riemann solutions (converged level)
+ *   - Ducros sensor: 1 for a pure dilatation, 0 for a pure rotation,
+ *     a^2/(a^2+b^2) for a mixed linear field; pressure factor 0 on linear p
+ *     and 24k h^2/(64 p0 + 40k h^2) on p = p0 + k x^2
+ *   - SSP-RK3 stage table: the three stages of orc_step driven by fluxes
+ *     whose residual is lambda*U multiply U by 1 + z + z^2/2 + z^3/6
+ *     (z = -lambda dt), and the alpha source likewise
+ *   - pressure floors: a floored state lands on rho c~^2 = 1e-8 (face state
+ *     and first-order flux)
+ *   - Wong limiter: theta lands the limited cell's forward-Euler state on
+ *     the floor through the blended faces, faces take the min of both
+ *     cells; no-op on admissible data; forced-failure repair
+ *   - whole step: free-stream preservation, y-mirror and x<->y transpose
+ *     symmetry, conservation with quiescent boundaries, CFL dt closed form
  *   The reference ships no golden vectors; the end-to-end trajectories of the
  *   composite flux kernels are pinned only by the restatement plus these
- *   invariants (see tests/golden/README for the oracle-generated anchors).
+ *   invariants.  The .npz files under tests/golden are oracle-generated
+ *   regression anchors (tests/golden/make_golden.py), not reference vectors.
  */
 #include <math.h>
 #include <stdint.h>
@@ -169,7 +182,7 @@ static inline double rho_ctilde2_sc(const orc_eos *e, S6 s)
 }
 
 /* SC:180-187 (used by M10:1015, 1021) */
-static inline S6 pp_floor_pressure_state(const orc_eos *e, S6 s)
+S6 orc_pp_floor_pressure_state(const orc_eos *e, S6 s)
 {
     double a1 = orc_clamp01_pp(s.v5);
     double g, pi;
@@ -656,6 +669,8 @@ struct orc_ctx {
     orc_params p;
     int nxg, nyg;
     double *a[A_COUNT];
+    orc_flux_hook flux_hook;   /* test hook: replaces the flux kernels in orc_step */
+    void *flux_user;
 };
 
 #define IDX(i, j) ((size_t)(i) * (size_t)nyg + (size_t)(j))
@@ -970,12 +985,12 @@ static void face_high(orc_ctx *c, int dir, int ii, int jj, double out[7])
         if (phL.v1 < 0.0 || isnan(phL.v1)) phL.v1 = phfL.v1;
         if (phL.v5 < 0.0 || phL.v5 > W1) phL.v5 = phfL.v5;
         if (rho_ctilde2_sc(e, phL) < WPP_CTILDE || isnan(phL.v4)) phL.v4 = phfL.v4;
-        phL = pp_floor_pressure_state(e, phL);
+        phL = orc_pp_floor_pressure_state(e, phL);
         if (phR.v0 < 0.0 || isnan(phR.v0)) phR.v0 = phfR.v0;
         if (phR.v1 < 0.0 || isnan(phR.v1)) phR.v1 = phfR.v1;
         if (phR.v5 < 0.0 || phR.v5 > W1) phR.v5 = phfR.v5;
         if (rho_ctilde2_sc(e, phR) < WPP_CTILDE || isnan(phR.v4)) phR.v4 = phfR.v4;
-        phR = pp_floor_pressure_state(e, phR);
+        phR = orc_pp_floor_pressure_state(e, phR);
     }
     if (ca < 0.0 || isnan(ca)) {
         phL = phfL;
@@ -1236,6 +1251,12 @@ void orc_prepare(orc_ctx *c)
     k_apply_bc(c);
 }
 
+void orc_set_flux_hook(orc_ctx *c, orc_flux_hook fn, void *user)
+{
+    c->flux_hook = fn;
+    c->flux_user = user;
+}
+
 /* One SSP-RK3 step, SC:2028-2055. */
 void orc_step(orc_ctx *c, double dt)
 {
@@ -1243,7 +1264,8 @@ void orc_step(orc_ctx *c, double dt)
     orc_copy_old(c);
     for (int s = 0; s < 3; ++s) {
         k_apply_bc(c);
-        k_fluxes(c);
+        if (c->flux_hook) c->flux_hook(c->flux_user, s);
+        else k_fluxes(c);
         k_limit_fluxes(c, dt);
         k_rk3(c, ab[s][0], ab[s][1], dt);
         orc_enforce_admissible(c);

@@ -51,6 +51,14 @@ void orc_rk3_update(orc_ctx *c, double alpha, double beta, double dt);
 void orc_copy_old(orc_ctx *c);
 double orc_get_dt(orc_ctx *c);
 void orc_step(orc_ctx *c, double dt);
+
+/* Test hook (no counterpart in the reference): while set, orc_step calls
+   fn(user, stage) for stage 0, 1, 2 in place of compute_fluxes +
+   compute_low_fluxes, so a test can drive the stage sequence (limiter,
+   SSP-RK3 combine, repair, prims) with face fluxes of its choosing.  NULL
+   restores the flux kernels. */
+typedef void (*orc_flux_hook)(void *user, int stage);
+void orc_set_flux_hook(orc_ctx *c, orc_flux_hook fn, void *user);
 void orc_run(orc_ctx *c, int nsteps, double *dts);
 
 /* scalar helpers (exported for known-answer tests) */
@@ -62,6 +70,7 @@ double orc_prim_to_energy(const orc_eos *e, double ar1, double ar2, double u,
 double orc_clamp01_pp(double a1);
 double orc_pp_theta_lower(double base, double high, double floor_);
 orc_s6 orc_pp_limit_reconstructed_state(const orc_eos *e, orc_s6 high, orc_s6 base);
+orc_s6 orc_pp_floor_pressure_state(const orc_eos *e, orc_s6 s);
 double orc_rho_ctilde2_cons(const orc_eos *e, double m1, double m2, double mx,
                             double my, double E, double a1_in);
 double orc_theta_cons_stage(const orc_eos *e, const double l[6], const double h[6], int stage);
