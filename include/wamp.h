@@ -215,6 +215,15 @@ int wamp_timing_read(wamp_ctx *ctx, double ms[2], int64_t launches[2]);
    doubles; out: 4n device doubles. */
 int wamp_selftest_arith(const double *a, const double *b, double *out, int64_t n, void *stream);
 
+/* Elementary-function self-test (no counterpart in the reference; for
+   tests).  For i < n: out[i] = x[i]^y[i] and out[n + i] = tanh(x[i]) as the
+   FP64 kernels evaluate them (the entropy's pow, cell_entropy SC:308-314;
+   THINC's tanh, SC:327-349): functions built from IEEE operations only, so a
+   host build of the same source gives the same bits.  x, y: n device
+   doubles; out: 2n device doubles. */
+int wamp_selftest_elementary(const double *x, const double *y, double *out, int64_t n,
+                             void *stream);
+
 /* ---- Three-dimensional extension (no counterpart in the reference, whose
    scripts are 2D; its README mentions 3D runs whose code it does not hold).
    The same step with a z-sweep and the 7-variable state [a1*rho1, a2*rho2,

@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 # the same source built with every double counted by operation kind (opcount.h)
 COUNT_LIB_PATH = os.path.join(HERE, "liborc_count.so")
+# the same source with the solver's own pow and tanh (shared_elementary.h)
+WM_LIB_PATH = os.path.join(HERE, "liborc_wm.so")
 OPCOUNT_KINDS = ("add", "mul", "div", "sqrt", "pow", "tanh", "fabs", "floor", "cmp")
 
 # array ids, mirror of the enum in wamp_oracle.c
@@ -50,10 +52,17 @@ class S6(C.Structure):
         return [self.v0, self.v1, self.v2, self.v3, self.v4, self.v5]
 
 
-def build(force: bool = False, counting: bool = False) -> str:
+def _path(counting: bool, shared_math: bool) -> str:
+    if counting and shared_math:
+        raise ValueError("the counting build uses the C library's pow and tanh")
+    return COUNT_LIB_PATH if counting else WM_LIB_PATH if shared_math else LIB_PATH
+
+
+def build(force: bool = False, counting: bool = False, shared_math: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, -O2 -ffp-contract=off); with
-    counting, the operation-counting build (g++, same flags, serial)."""
-    path = COUNT_LIB_PATH if counting else LIB_PATH
+    counting, the operation-counting build (g++, same flags, serial); with
+    shared_math, the build whose pow and tanh are the solver's own."""
+    path = _path(counting, shared_math)
     if force and os.path.exists(path):
         os.remove(path)
     subprocess.run(["make", "-s", "-C", HERE, os.path.basename(path)], check=True)
@@ -63,11 +72,11 @@ def build(force: bool = False, counting: bool = False) -> str:
 _libs = {}
 
 
-def lib(counting: bool = False):
-    path = COUNT_LIB_PATH if counting else LIB_PATH
+def lib(counting: bool = False, shared_math: bool = False):
+    path = _path(counting, shared_math)
     if path not in _libs:
         if not os.path.exists(path):
-            build(counting=counting)
+            build(counting=counting, shared_math=shared_math)
         L = C.CDLL(path)
         d, pd, ip = C.c_double, C.POINTER(C.c_double), C.c_void_p
         pe = C.POINTER(OrcEos)
@@ -91,7 +100,8 @@ def lib(counting: bool = False):
         # scalar helpers
         L.orc_mix_gamma_pibig.argtypes = [pe, d, pd, pd]
         L.orc_mix_gamma_pinf.argtypes = [pe, d, pd, pd]
-        for n, args in (("orc_sound_speed", [pe, d, d, d]),
+        for n, args in (("orc_pow", [d, d]), ("orc_tanh", [d]),
+                        ("orc_sound_speed", [pe, d, d, d]),
                         ("orc_prim_to_energy", [pe, d, d, d, d, d, d]),
                         ("orc_clamp01_pp", [d]),
                         ("orc_pp_theta_lower", [d, d, d]),
@@ -139,7 +149,8 @@ def dvec(values) -> "C.Array":
 class Oracle:
     """One oracle solver context over the reference's 47 padded arrays."""
 
-    def __init__(self, cfg, threads: int | None = None, counting: bool = False):
+    def __init__(self, cfg, threads: int | None = None, counting: bool = False,
+                 shared_math: bool = False):
         g = cfg.grid
         s = cfg.scheme
         p = OrcParams()
@@ -153,7 +164,7 @@ class Oracle:
         self.params = p
         self.cfg = cfg
         self.shape = (g.nxg, g.nyg)
-        self._L = lib(counting)
+        self._L = lib(counting, shared_math)
         self.counting = counting
         if threads is not None:
             self._L.orc_set_threads(int(threads))

@@ -65,6 +65,14 @@
 
 #include "wamp_oracle.h"
 
+/* pow and tanh as the step uses them (cell_entropy SC:308-314, THINC
+   SC:327-349), exported for known-answer tests.  liborc.so takes them from
+   the C library; liborc_wm.so is the same source built with the solver's
+   own pow and tanh (oracle/shared_elementary.h), so that it and the FP64 HIP
+   step differ in no elementary function. */
+double orc_pow(double x, double y) { return pow(x, y); }
+double orc_tanh(double x) { return tanh(x); }
+
 /* ---- constants, SC:44-70 ------------------------------------------------ */
 #define W1     1.0
 #define W0p5   0.5

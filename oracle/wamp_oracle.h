@@ -62,6 +62,8 @@ void orc_set_flux_hook(orc_ctx *c, orc_flux_hook fn, void *user);
 void orc_run(orc_ctx *c, int nsteps, double *dts);
 
 /* scalar helpers (exported for known-answer tests) */
+double orc_pow(double x, double y);
+double orc_tanh(double x);
 void orc_mix_gamma_pibig(const orc_eos *e, double a1, double *gb, double *pb);
 void orc_mix_gamma_pinf(const orc_eos *e, double a1, double *g, double *pi);
 double orc_sound_speed(const orc_eos *e, double rho, double p, double a1);

@@ -70,6 +70,41 @@ def test_invariant_3d_state_steps_exactly_as_2d(gpu_ok, scheme, embed):
                         f"max |diff| {d[k]:.3e} at {k} ({a[k]!r} vs {b[k]!r})")
 
 
+@pytest.mark.parametrize("embed", ["xy", "xz"])
+def test_invariant_3d_state_against_oracle(gpu_ok, embed):
+    """The embedded 2D shock-cylinder case (MP5), 10 steps of the 3D FP64
+    solver with the oracle's dt sequence, against the oracle's 2D state on
+    every interior slice."""
+    from conftest import rel_l1
+    from oracle.oracle import Oracle
+    from wamp import cases, threed
+    from wamp.config import SCHEME_MP5
+
+    cfg2, U2 = cases.shock_cylinder(64, 32, scheme=SCHEME_MP5)
+    cfg3, U3 = (threed.from_2d_xy if embed == "xy" else threed.from_2d_xz)(cfg2, U2, 4)
+    orc = Oracle(cfg2)
+    orc.set_state(U2)
+    orc.prepare()
+    s3 = threed.Solver3(cfg3)
+    s3.set_state(U3)
+    s3.prepare()
+    for _ in range(10):
+        dt = orc.get_dt()
+        orc.step(dt)
+        s3.step_with(dt)
+    ref, got = orc.state(), s3.state()
+    orc.close()
+    s3.close()
+    worst = 0.0
+    for name, a, b in _slices(cfg2, ref, cfg3, got, embed):
+        if np.abs(b).sum() > 0:
+            worst = max(worst, rel_l1(a, b))
+        else:
+            assert np.all(a == 0.0), name
+    print(f"3D {embed} embedding vs oracle after 10 steps: max rel L1 {worst:.2e}")
+    assert worst <= 1e-12
+
+
 def test_one_step_reduction_on_a_ragged_grid(gpu_ok):
     """Grid sizes that are not multiples of the launch tiles (37 x 23)."""
     cfg2, U2, cfg3, U3 = _pair("mp5", "xz", steps=1, nx=37, ny=23)

@@ -111,3 +111,39 @@ def test_selftest_rejects_bad_arguments(gpu_ok):
     assert lib.wamp_selftest_arith(None, None, None, 0, None) == 0
     assert lib.wamp_selftest_arith(None, None, None, 4, None) != 0
     assert lib.wamp_selftest_arith(None, None, None, -1, None) != 0
+
+
+def test_elementary_functions_are_the_host_build_bit_for_bit(gpu_ok):
+    """pow and tanh of the FP64 kernels (wamp_math.h) against the same source
+    compiled by gcc into the oracle build liborc_wm.so: identical bits over
+    the step's argument ranges and beyond (tests/test_math.py checks that
+    source against 60-digit arithmetic)."""
+    import torch
+
+    from oracle import oracle as O
+    from wamp import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(9)
+    n = 200_000
+    x = np.concatenate([10.0 ** rng.uniform(-20, 6, n // 2), rng.uniform(-30.0, 30.0, n // 4),
+                        rng.uniform(-0.9, 0.9, n // 4)])
+    y = np.concatenate([rng.uniform(1.0, 7.0, n // 2), rng.uniform(-3.0, 3.0, n // 2)])
+    x[:8] = [0.0, -0.0, 1e-300, 5e-324, 1.0, 0.9, -0.9, 1e-20]
+    out = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    dx, dy = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    _lib.check(lib.wamp_selftest_elementary(dx.data_ptr(), dy.data_ptr(), out.data_ptr(), n,
+                                            torch.cuda.current_stream().cuda_stream),
+               "wamp_selftest_elementary")
+    got = out.cpu().numpy()
+    W = O.lib(shared_math=True)
+    px = np.abs(x)   # pow's domain in the step: positive bases
+    want_pow = np.array([W.orc_pow(float(a), float(b)) for a, b in zip(px, y)])
+    dpx = torch.from_numpy(px).cuda()   # held until the kernel has run
+    _lib.check(lib.wamp_selftest_elementary(dpx.data_ptr(), dy.data_ptr(), out.data_ptr(), n,
+                                            torch.cuda.current_stream().cuda_stream),
+               "wamp_selftest_elementary")
+    got_pow = out[:n].cpu().numpy()
+    want_tanh = np.array([W.orc_tanh(float(a)) for a in x])
+    assert np.array_equal(got_pow.view(np.uint64), want_pow.view(np.uint64))
+    assert np.array_equal(got[n:].view(np.uint64), want_tanh.view(np.uint64))

@@ -17,6 +17,7 @@
 
 #include "wamp.h"
 
+#include "wamp_math.h"
 #include "wamp_types.h"
 
 using namespace wamp;
@@ -41,6 +42,8 @@ hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, h
 hipError_t wamp_launch_fill(double* p, double v, hipStream_t);
 hipError_t wamp_launch_selftest_arith(const double* a, const double* b, double* out, long long n,
                                       hipStream_t);
+hipError_t wamp_launch_selftest_elementary(const double* x, const double* y, double* out,
+                                           long long n, hipStream_t);
 hipError_t wamp_launch_prepare(const Geo*, const Eos*, const State*, int repair, hipStream_t);
 hipError_t wamp_launch_enforce(const Geo*, const Eos*, const State*, hipStream_t);
 hipError_t wamp_launch_cons_to_prim(const Geo*, const Eos*, const State*, double*, double*,
@@ -188,7 +191,7 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   c->eos = make_eos(*eos);
   c->scheme = *scheme;
   c->precision = precision;
-  c->t1 = std::tanh(1.8 * 0.5);
+  c->t1 = wm_tanh(1.8 * 0.5);
   c->lu_kind = -1;
   c->lu_seg = 0;
   if (const char* v = std::getenv("WAMP_LU_KIND")) c->lu_kind = std::atoi(v);
@@ -463,6 +466,15 @@ int wamp_selftest_arith(const double* a, const double* b, double* out, int64_t n
                    "selftest_arith");
 }
 
+int wamp_selftest_elementary(const double* x, const double* y, double* out, int64_t n,
+                             void* stream) {
+  if (n < 0 || n > (int64_t)1 << 30) return fail(-3, "n must be in [0, 2^30]");
+  if (n == 0) return 0;
+  if (!x || !y || !out) return fail(-2, "x, y and out must be non-NULL");
+  return hip_check(wamp_launch_selftest_elementary(x, y, out, (long long)n, (hipStream_t)stream),
+                   "selftest_elementary");
+}
+
 int wamp_timing_enable(wamp_ctx* c, int32_t max_steps) {
   if (int rc = check_ctx(c)) return rc;
   if (max_steps < 0 || max_steps > 100000) return fail(-3, "max_steps must be in [0, 100000]");
@@ -580,7 +592,7 @@ int wamp_batch_create(const wamp_scheme* scheme, int32_t precision, int32_t max_
   b->scheme = *scheme;
   b->precision = precision;
   b->max_cases = max_cases;
-  b->t1 = std::tanh(1.8 * 0.5);
+  b->t1 = wm_tanh(1.8 * 0.5);
   b->tile = 0;
   b->faces_bpcu = 3;
   b->lu_bpcu = 2;
@@ -917,7 +929,7 @@ int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_sch
   c->eos = make_eos(*eos);
   c->scheme = *scheme;
   c->precision = precision;
-  c->t1 = std::tanh(1.8 * 0.5);
+  c->t1 = wm_tanh(1.8 * 0.5);
   const size_t plane = (size_t)plane64;
   // face planes: 48 FP64, or in mixed precision 6 FP64 energy planes and 42
   // FP32 planes (a multiple of 8 bytes: 42 * 4 * plane)

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wamp_math.h"
 #include "wamp_types.h"
 
 // The FP64 kernels (wamp_kernels.hip) and the FP32-mixed face kernels
@@ -139,11 +140,17 @@ __device__ __forceinline__ double sq(double x) {
 #endif
 }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
-// FP64 tanh (THINC) out of line: the library tanh is ~170 instructions and
-// the face kernel has up to 12 THINC sites; one shared body keeps the M10
-// FP64 face kernel at 44 KB of code instead of 60 KB (the instruction cache
-// is 64 KB per CU pair).  It runs only where THINC fires.
+// FP64 tanh (THINC) out of line: the face kernel has up to 12 THINC sites;
+// one shared body keeps the FP64 face kernel's code within the 64 KB
+// instruction cache of a CU pair.  It runs only where THINC fires.  The FP64
+// kernels use wm_tanh (wamp_math.h: IEEE operations only, so host builds of
+// the same source give the same bits); the mixed unit keeps the library tanh
+// for its few FP64 THINC values.
+#ifdef WAMP_MIXED_TU
 __device__ __noinline__ double th(double x) { return tanh(x); }
+#else
+__device__ __noinline__ double th(double x) { return wm_tanh(x); }
+#endif
 __device__ __forceinline__ float th(float x) {
   float t = __expf(-2.0f * fabsf(x));
   float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
@@ -330,7 +337,8 @@ __device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, 
   // bases; the indicator built on it only gates the reconstruction.
   return ddiv(p, exp(g * log(rho)));
 #else
-  return ddiv(p, pow(rho, g));
+  // FP64 kernels: wm_pow (wamp_math.h), within one ulp of rho^g
+  return ddiv(p, wm_pow(rho, g));
 #endif
 }
 // cator2_val SC:318-325

@@ -708,6 +708,16 @@ __global__ void __launch_bounds__(256) k_selftest_arith(const double* __restrict
   out[3 * n + i] = sqrt(y);
 }
 
+// the FP64 kernels' pow and tanh (wamp_math.h) element by element
+__global__ void __launch_bounds__(256) k_selftest_elementary(const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             double* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  out[i] = wm_pow(x[i], y[i]);
+  out[n + i] = th(x[i]);
+}
+
 // ========================================================================
 // small kernels
 // ========================================================================
@@ -1066,6 +1076,13 @@ hipError_t wamp_launch_dt(const Geo* G, const Eos* e, const State* U, double* dt
 
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t st) {
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, dt_acc, dt_out, cfl);
+  return hipGetLastError();
+}
+
+hipError_t wamp_launch_selftest_elementary(const double* x, const double* y, double* out,
+                                           long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest_elementary, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x,
+                     y, out, n);
   return hipGetLastError();
 }
 

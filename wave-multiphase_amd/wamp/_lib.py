@@ -94,6 +94,7 @@ SIGNATURES = [
     ("wamp_timing_enable", C.c_int, [_VP, C.c_int32]),
     ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("wamp_selftest_arith", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
+    ("wamp_selftest_elementary", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
     ("wamp3_ctx_create", C.c_int, [C.POINTER(WampGrid3), C.POINTER(WampEos), C.POINTER(WampScheme),
                                    C.c_int32, C.POINTER(_VP)]),
     ("wamp3_ctx_destroy", C.c_int, [_VP]),

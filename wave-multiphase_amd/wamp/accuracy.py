@@ -41,16 +41,18 @@ def compare(cfg: CaseConfig, U, Uref) -> dict:
     """Errors of state U against the reference state Uref, both (6, NXG, NYG) FP64."""
     g = cfg.grid
     si, sj = g.interior()
-    rho, rho_r = (U[0] + U[1])[si, sj], (Uref[0] + Uref[1])[si, sj]
-    p, p_r = _pressure(cfg, U)[si, sj], _pressure(cfg, Uref)[si, sj]
+    # interior cells only: the ghost layers may hold zeros (no density to divide by)
+    Ui, Ri = U[:, si, sj], Uref[:, si, sj]
+    rho, rho_r = Ui[0] + Ui[1], Ri[0] + Ri[1]
+    p, p_r = _pressure(cfg, Ui), _pressure(cfg, Ri)
     cx, cy = centroid(cfg, U)
     rx, ry = centroid(cfg, Uref)
     return {
         "rho": rel_l1(rho, rho_r),
         "p": rel_l1(p, p_r),
-        "alpha1": rel_l1(U[5][si, sj], Uref[5][si, sj]),
+        "alpha1": rel_l1(Ui[5], Ri[5]),
         "centroid_shift_cells": float(np.hypot((cx - rx) / g.dx, (cy - ry) / g.dy)),
-        "finite": bool(np.isfinite(U[:, si, sj]).all()),
+        "finite": bool(np.isfinite(Ui).all()),
     }
 
 
