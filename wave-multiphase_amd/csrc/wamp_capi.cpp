@@ -17,6 +17,7 @@
 
 #include "wamp.h"
 
+#include "wamp_internal.h"
 #include "wamp_math.h"
 #include "wamp_types.h"
 
@@ -39,7 +40,6 @@ hipError_t wamp_launch_lu_march(const Geo*, const Eos*, const State* Un, const S
 hipError_t wamp_launch_dt(const Geo*, const Eos*, const State*, double* dt_acc, double* dt_out,
                           double cfl, hipStream_t);
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t);
-hipError_t wamp_launch_fill(double* p, double v, hipStream_t);
 hipError_t wamp_launch_selftest_arith(const double* a, const double* b, double* out, long long n,
                                       hipStream_t);
 hipError_t wamp_launch_selftest_elementary(const double* x, const double* y, double* out,
@@ -81,21 +81,11 @@ hipError_t wamp3_launch_dt(const Geo3*, const Eos*, const State3*, double* dt_ac
 hipError_t wamp3_launch_prepare(const Geo3*, const Eos*, const State3*, int repair, hipStream_t);
 }
 
-struct wamp_ctx {
-  Geo geo;
-  Eos eos;
-  wamp_scheme scheme;
-  int precision;
-  int device;
-  double t1;          // tanh(beta_thinc / 2), SC:334
-  int lu_kind;        // 0: LDS-tile limiter/update kernel, 1: marching kernel, -1: by grid size
-  int lu_seg;         // rows per wave of the marching kernel (0: by grid size)
-  int cu_count;       // compute units of the device (limiter segment choice)
+// A context is a step plan plus the scratch it owns.
+struct wamp_ctx : StepPlan {
   double* block;      // one allocation: faces | stage buffers | dt accumulator
   int64_t bytes;
-  Faces faces;
-  State u1, u2;
-  double* dt_acc;
+  StepScratch scr;
   // measurement hook: events around the kernels of up to timing_cap steps
   int timing_cap;
   int timing_n;
@@ -106,8 +96,9 @@ static constexpr int EV_PER_STEP = 7;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
-static int fail(int code, const char* fmt, ...) {
+namespace wamp {
+
+int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -117,12 +108,12 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-static int hip_check(hipError_t e, const char* what) {
+int hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
   return fail((int)e, "%s: %s", what, hipGetErrorString(e));
 }
 
-static int state_of(double* const U[6], State& s, const char* name) {
+int state_of(double* const U[6], State& s, const char* name) {
   if (!U) return fail(-2, "%s: NULL array list", name);
   for (int m = 0; m < 6; ++m) {
     if (!U[m]) return fail(-2, "%s[%d] is NULL", name, m);
@@ -131,24 +122,15 @@ static int state_of(double* const U[6], State& s, const char* name) {
   return 0;
 }
 
-static int check_ctx(const wamp_ctx* c) {
-  if (!c) return fail(-1, "NULL context");
-  return 0;
-}
-
-extern "C" {
-
-const char* wamp_last_error(void) { return g_err.c_str(); }
-
 // Bytes of the face planes of one case: 28 FP64 planes, or, in mixed
 // precision, the four FP64 energy planes and 24 FP32 planes (the mixed face
 // kernels' other fluxes are FP32 values; see Faces).
-static size_t face_bytes(size_t plane, int precision) {
+size_t face_bytes(size_t plane, int precision) {
   return precision == 0 ? 28 * plane * sizeof(double)
                         : 4 * plane * sizeof(double) + 24 * plane * sizeof(float);
 }
 // Faces over `bytes = face_bytes(plane, precision)` at p (8-byte aligned).
-static Faces carve_faces(char* p, size_t plane, int precision) {
+Faces carve_faces(char* p, size_t plane, int precision) {
   Faces f{(double*)p, (long long)plane};
   if (precision != 0) {
     f.h = (float*)(p + 4 * plane * sizeof(double));
@@ -156,11 +138,21 @@ static Faces carve_faces(char* p, size_t plane, int precision) {
   }
   return f;
 }
+size_t step_scratch_bytes(size_t plane, int precision) {
+  return face_bytes(plane, precision) + (12 * plane + 32) * sizeof(double);
+}
+StepScratch carve_step_scratch(void* base, size_t plane, int precision) {
+  StepScratch s;
+  s.faces = carve_faces((char*)base, plane, precision);
+  double* p = (double*)((char*)base + face_bytes(plane, precision));  // a multiple of 8
+  for (int m = 0; m < 6; ++m, p += plane) s.u1.f[m] = p;
+  for (int m = 0; m < 6; ++m, p += plane) s.u2.f[m] = p;
+  s.dt_acc = p;
+  return s;
+}
 
-int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_scheme* scheme,
-                    int32_t precision, wamp_ctx** out) {
-  if (!out) return fail(-1, "out is NULL");
-  *out = nullptr;
+int check_case(const wamp_grid* grid, const wamp_eos* eos, const wamp_scheme* scheme,
+               int32_t precision) {
   if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
   if (grid->nx < 1 || grid->ny < 1) return fail(-3, "nx, ny must be >= 1 (got %d, %d)", grid->nx, grid->ny);
   if (grid->g < 4) return fail(-3, "g must be >= 4 (the face stencil reaches 4 cells), got %d", grid->g);
@@ -178,42 +170,64 @@ int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_schem
   if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
   if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
     return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
+  return 0;
+}
 
+void plan_fill(StepPlan& p, const wamp_grid& grid, const wamp_eos& eos, const wamp_scheme& scheme,
+               int precision, int device, int cu_count) {
+  p.geo.nx = grid.nx;
+  p.geo.ny = grid.ny;
+  p.geo.g = grid.g;
+  p.geo.ld = grid.ld;
+  p.geo.dx = grid.dx;
+  p.geo.dy = grid.dy;
+  for (int k = 0; k < 4; ++k) p.geo.bc[k] = grid.bc[k];
+  p.eos = make_eos(eos);
+  p.scheme = scheme;
+  p.precision = precision;
+  p.device = device;
+  p.t1 = wm_tanh(1.8 * 0.5);
+  p.lu_kind = -1;
+  p.lu_seg = 0;
+  if (const char* v = std::getenv("WAMP_LU_KIND")) p.lu_kind = std::atoi(v);
+  if (const char* v = std::getenv("WAMP_LU_SEG")) p.lu_seg = std::atoi(v);
+  p.cu_count = cu_count;
+}
+
+}  // namespace wamp
+
+static int check_ctx(const wamp_ctx* c) {
+  if (!c) return fail(-1, "NULL context");
+  return 0;
+}
+
+extern "C" {
+
+const char* wamp_last_error(void) { return g_err.c_str(); }
+
+
+int wamp_ctx_create(const wamp_grid* grid, const wamp_eos* eos, const wamp_scheme* scheme,
+                    int32_t precision, wamp_ctx** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (int rc = check_case(grid, eos, scheme, precision)) return rc;
+  int device = 0, cu_count = 0;
+  int rc = hip_check(hipGetDevice(&device), "hipGetDevice");
+  if (!rc)
+    rc = hip_check(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, device),
+                   "hipDeviceGetAttribute");
+  if (rc) return rc;
   wamp_ctx* c = new (std::nothrow) wamp_ctx();
   if (!c) return fail(-5, "host allocation failed");
-  c->geo.nx = grid->nx;
-  c->geo.ny = grid->ny;
-  c->geo.g = grid->g;
-  c->geo.ld = grid->ld;
-  c->geo.dx = grid->dx;
-  c->geo.dy = grid->dy;
-  for (int k = 0; k < 4; ++k) c->geo.bc[k] = grid->bc[k];
-  c->eos = make_eos(*eos);
-  c->scheme = *scheme;
-  c->precision = precision;
-  c->t1 = wm_tanh(1.8 * 0.5);
-  c->lu_kind = -1;
-  c->lu_seg = 0;
-  if (const char* v = std::getenv("WAMP_LU_KIND")) c->lu_kind = std::atoi(v);
-  if (const char* v = std::getenv("WAMP_LU_SEG")) c->lu_seg = std::atoi(v);
-  int rc = hip_check(hipGetDevice(&c->device), "hipGetDevice");
-  if (!rc)
-    rc = hip_check(hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount,
-                                         c->device), "hipDeviceGetAttribute");
-  if (rc) { delete c; return rc; }
+  plan_fill(*c, *grid, *eos, *scheme, precision, device, cu_count);
 
   const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
-  const size_t fbytes = face_bytes(plane, precision);  // a multiple of 8
-  const size_t bytes = fbytes + (12 * plane + 32) * sizeof(double);
+  const size_t bytes = step_scratch_bytes(plane, precision);
   rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(scratch)");
   if (rc) { delete c; return rc; }
   c->bytes = (int64_t)bytes;
-  c->faces = carve_faces((char*)c->block, plane, precision);
-  double* p = (double*)((char*)c->block + fbytes);
-  for (int m = 0; m < 6; ++m, p += plane) c->u1.f[m] = p;
-  for (int m = 0; m < 6; ++m, p += plane) c->u2.f[m] = p;
-  c->dt_acc = p;
-  rc = hip_check(wamp_launch_fill(c->dt_acc, 1.0e30, nullptr), "fill dt accumulator");
+  c->scr = carve_step_scratch(c->block, plane, precision);
+  rc = hip_check(wamp_launch_fill(c->scr.dt_acc, 1.0e30, nullptr), "fill dt accumulator");
   if (!rc) rc = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   if (rc) {
     (void)hipFree(c->block);
@@ -306,7 +320,7 @@ static int faces_of(const wamp_ctx* c, double* const F[28], Faces& f) {
 // 0.111), 2048x1024 (20: 0.231; 16: 0.311) and 4096x2048 (26: 0.861; 16:
 // 0.916).
 constexpr int MJ_COLS = 58;  // output columns per limiter wave (64 lanes - 2 x 3 halo)
-static int lu_rows(const wamp_ctx* c) {
+static int lu_rows(const StepPlan* c) {
   if (c->lu_seg) return c->lu_seg;
   static const int cand[] = {4, 8, 16, 20, 26, 28, 32};
   const long long strips = (c->geo.ny + MJ_COLS - 1) / MJ_COLS;
@@ -330,12 +344,12 @@ static int lu_rows(const wamp_ctx* c) {
 // SIMDs idle; the marching kernel moves fewer bytes and wins from about
 // 2^18 cells (measured: 512x256 0.12 vs 0.19 ms per step, 1024x512 0.37 vs
 // 0.33, 2048x1024 1.45 vs 1.14).  Both give bitwise the same result.
-static int lu_kind(const wamp_ctx* c) {
+static int lu_kind(const StepPlan* c) {
   if (c->lu_kind >= 0) return c->lu_kind;
   return (long long)c->geo.nx * c->geo.ny <= (1LL << 18) ? 0 : 1;
 }
 
-static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& f,
+static int launch_lu(const StepPlan* c, const State& un, const State& u, const Faces& f,
                      const State& uo, double alpha, double beta, const double* dt_dev,
                      double* dt_acc, hipStream_t st) {
   hipError_t e = lu_kind(c) == 1
@@ -346,7 +360,7 @@ static int launch_lu(wamp_ctx* c, const State& un, const State& u, const Faces& 
   return hip_check(e, "limit_update");
 }
 
-static int launch_faces(wamp_ctx* c, const State& s, const Faces& f, hipStream_t st) {
+static int launch_faces(const StepPlan* c, const State& s, const Faces& f, hipStream_t st) {
   return hip_check(wamp_launch_faces(&c->geo, &c->eos, &s, &f, c->scheme.base, c->scheme.shear,
                                      c->scheme.face_pp, c->scheme.vf_thinc, c->t1, c->precision, st),
                    "faces");
@@ -382,7 +396,7 @@ int wamp_get_dt(wamp_ctx* c, double* const U[6], double* dt_dev, void* stream) {
   State s;
   if (int rc = state_of(U, s, "U")) return rc;
   if (!dt_dev) return fail(-2, "dt_dev is NULL");
-  return hip_check(wamp_launch_dt(&c->geo, &c->eos, &s, c->dt_acc, dt_dev, c->scheme.cfl,
+  return hip_check(wamp_launch_dt(&c->geo, &c->eos, &s, c->scr.dt_acc, dt_dev, c->scheme.cfl,
                                   (hipStream_t)stream), "get_dt");
 }
 
@@ -392,14 +406,20 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
   State a;
   if (int rc = state_of(U, a, "U")) return rc;
   if (!dt_dev || !dt_next_dev) return fail(-2, "dt_dev and dt_next_dev must be non-NULL");
-  hipStream_t st = (hipStream_t)stream;
   hipEvent_t* ev = nullptr;
   if (c->timing_n < c->timing_cap) ev = c->ev + (size_t)c->timing_n++ * EV_PER_STEP;
+  return step_enqueue(*c, a, c->scr, dt_dev, dt_next_dev, (hipStream_t)stream, ev);
+}
+
+}  // extern "C"
+
+int wamp::step_enqueue(const StepPlan& p, const State& a, const StepScratch& scr,
+                       const double* dt_dev, double* dt_next_dev, hipStream_t st, hipEvent_t* ev) {
   auto mark = [&](int k) -> int {
     return ev ? hip_check(hipEventRecord(ev[k], st), "hipEventRecord") : 0;
   };
-  const State* cur[3] = {&a, &c->u1, &c->u2};
-  const State* dst[3] = {&c->u1, &c->u2, &a};
+  const State* cur[3] = {&a, &scr.u1, &scr.u2};
+  const State* dst[3] = {&scr.u1, &scr.u2, &a};
   const double coef[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
   // stage 1: U1 = Un + dt R(Un)                               (SC:2031-2037)
   // stage 2: U2 = 3/4 Un + 1/4 (U1 + dt R(U1))                (SC:2040-2046)
@@ -407,15 +427,17 @@ int wamp_step(wamp_ctx* c, double* const U[6], const double* dt_dev, double* dt_
   //          CFL minimum of the new state folded in           (SC:2049-2059)
   if (int rc = mark(0)) return rc;
   for (int s = 0; s < 3; ++s) {
-    if (int rc = launch_faces(c, *cur[s], c->faces, st)) return rc;
+    if (int rc = launch_faces(&p, *cur[s], scr.faces, st)) return rc;
     if (int rc = mark(2 * s + 1)) return rc;
-    if (int rc = launch_lu(c, a, *cur[s], c->faces, *dst[s], coef[s][0], coef[s][1], dt_dev,
-                           s == 2 ? c->dt_acc : nullptr, st))
+    if (int rc = launch_lu(&p, a, *cur[s], scr.faces, *dst[s], coef[s][0], coef[s][1], dt_dev,
+                           s == 2 ? scr.dt_acc : nullptr, st))
       return rc;
     if (int rc = mark(2 * s + 2)) return rc;
   }
-  return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, c->scheme.cfl, st), "dt");
+  return hip_check(wamp_launch_dt_finalize(scr.dt_acc, dt_next_dev, p.scheme.cfl, st), "dt");
 }
+
+extern "C" {
 
 // ---- initial conditions on the device ------------------------------------
 static int geo_of(const wamp_grid* grid, Geo& G) {
@@ -657,17 +679,21 @@ int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, cons
   if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
   State st;
   if (int rc = state_of(U, st, "U")) return rc;
-  if (b->scratch[slot]) {
-    (void)hipFree(b->scratch[slot]);
-    b->scratch[slot] = nullptr;
-  }
   const size_t plane = (size_t)(grid->nx + 2 * grid->g) * (size_t)grid->ld;
   const size_t fbytes = face_bytes(plane, b->precision);
+  // the new scratch first: if it cannot be had, the slot keeps its case and
+  // the batch stays as it was (still committed, graph still valid)
   char* base = nullptr;
   if (int rc = hip_check(hipMalloc((void**)&base, fbytes + 12 * plane * sizeof(double)),
                          "hipMalloc(case scratch)"))
     return rc;
+  if (b->scratch[slot]) {
+    // the old scratch may still be in use by enqueued work of the batch
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->scratch[slot]);
+  }
   b->scratch[slot] = (double*)base;
+  graph_drop(b);  // a captured graph holds the old case table
   CaseDesc& d = b->desc[slot];
   d = CaseDesc();
   d.geo = Geo{grid->nx, grid->ny, grid->g, grid->ld, grid->dx, grid->dy,
@@ -689,7 +715,10 @@ int wamp_batch_set_case(wamp_batch* b, int32_t slot, const wamp_grid* grid, cons
 int wamp_batch_clear_case(wamp_batch* b, int32_t slot) {
   if (!b) return fail(-1, "NULL batch");
   if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
-  if (b->scratch[slot]) (void)hipFree(b->scratch[slot]);
+  if (b->scratch[slot]) {
+    (void)hipDeviceSynchronize();  // enqueued work of the batch may still use it
+    (void)hipFree(b->scratch[slot]);
+  }
   b->scratch[slot] = nullptr;
   b->used[slot] = 0;
   b->desc[slot] = CaseDesc();

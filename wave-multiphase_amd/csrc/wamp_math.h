@@ -12,10 +12,13 @@
  * within a few hundred steps; with these two functions the FP64 step is a
  * fixed sequence of correctly rounded operations.
  *
+ * Divisions go through wm_div (below); the series are evaluated by Horner's
+ * rule with explicit fused multiply-adds (correctly rounded everywhere).
+ *
  * Methods (both from the mathematics, no table):
  *  - log in double-double: x = 2^k m, m in [1/sqrt2, sqrt2); ln m =
  *    2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172, s carried as a
- *    double-double and the odd series s^3/3 + ... + s^27/27 in double;
+ *    double-double and the odd series s^3/3 + ... + s^25/25 in double;
  *    k ln 2 with ln 2 split into two doubles.
  *  - exp of a double-double z: z = n ln2 + r, |r| <= ln2/2 (Cody-Waite
  *    reduction with a 32-bit head of ln 2, so n * head is exact), e^r by its
@@ -52,6 +55,27 @@ WM_FN double wm_double(uint64_t u) { double x; __builtin_memcpy(&x, &u, 8); retu
 /* 2^n for -1022 <= n <= 1023 */
 WM_FN double wm_pow2(int n) { return wm_double((uint64_t)(n + 1023) << 52); }
 
+/* a / b, correctly rounded.  In device code: the hardware reciprocal, two
+ * Newton steps and one correction (Markstein), which is the IEEE quotient
+ * bit for bit for finite b with 2^-1000 <= |b| <= 2^1000, a = 0 or |a| >=
+ * 2^-960 and |a / b| in [2^-1000, 2^700] (a zero quotient may carry the
+ * other sign); every division below stays in that range (tests/test_gpu_arith.py
+ * checks the sequence against IEEE division).  Host code divides. */
+#if defined(__HIP_DEVICE_COMPILE__)
+WM_FN double wm_div(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
+#else
+WM_FN double wm_div(double a, double b) { return a / b; }
+#endif
+
 /* s + e = a + b exactly (Knuth) */
 WM_FN double wm_two_sum(double a, double b, double *e)
 {
@@ -86,23 +110,21 @@ WM_FN double wm_log_dd(double x, double *lo)
     const double num = m - 1.0;    /* exact (Sterbenz) */
     double dl;
     const double dh = wm_two_sum(m, 1.0, &dl);
-    const double sh = num / dh;
+    const double sh = wm_div(num, dh);
     const double r = fma(-sh, dh, num) - sh * dl;
-    const double sl = r / dh;      /* sh + sl = num / (dh + dl) */
+    const double sl = wm_div(r, dh);   /* sh + sl = num / (dh + dl) */
     const double s2 = sh * sh;
-    double t = 1.0 / 27.0;
-    t = t * s2 + 1.0 / 25.0;
-    t = t * s2 + 1.0 / 23.0;
-    t = t * s2 + 1.0 / 21.0;
-    t = t * s2 + 1.0 / 19.0;
-    t = t * s2 + 1.0 / 17.0;
-    t = t * s2 + 1.0 / 15.0;
-    t = t * s2 + 1.0 / 13.0;
-    t = t * s2 + 1.0 / 11.0;
-    t = t * s2 + 1.0 / 9.0;
-    t = t * s2 + 1.0 / 7.0;
-    t = t * s2 + 1.0 / 5.0;
-    t = t * s2 + 1.0 / 3.0;
+    double t = fma(1.0 / 25.0, s2, 1.0 / 23.0);
+    t = fma(t, s2, 1.0 / 21.0);
+    t = fma(t, s2, 1.0 / 19.0);
+    t = fma(t, s2, 1.0 / 17.0);
+    t = fma(t, s2, 1.0 / 15.0);
+    t = fma(t, s2, 1.0 / 13.0);
+    t = fma(t, s2, 1.0 / 11.0);
+    t = fma(t, s2, 1.0 / 9.0);
+    t = fma(t, s2, 1.0 / 7.0);
+    t = fma(t, s2, 1.0 / 5.0);
+    t = fma(t, s2, 1.0 / 3.0);
     const double tail = sh * (s2 * t);            /* s^3/3 + s^5/5 + ... */
     double ml;
     const double mh = wm_fast_two_sum(2.0 * sh, 2.0 * (sl + tail), &ml);  /* ln m */
@@ -124,18 +146,17 @@ WM_FN double wm_exp_dd(double zh, double zl, int *n_out, double *lo)
     double re;
     const double rh = wm_two_sum(r1, -(nd * WM_L2L), &re);
     const double rl = re + zl;
-    double q = 1.0 / 6227020800.0;                             /* 1/13! */
-    q = q * rh + 1.0 / 479001600.0;
-    q = q * rh + 1.0 / 39916800.0;
-    q = q * rh + 1.0 / 3628800.0;
-    q = q * rh + 1.0 / 362880.0;
-    q = q * rh + 1.0 / 40320.0;
-    q = q * rh + 1.0 / 5040.0;
-    q = q * rh + 1.0 / 720.0;
-    q = q * rh + 1.0 / 120.0;
-    q = q * rh + 1.0 / 24.0;
-    q = q * rh + 1.0 / 6.0;
-    q = q * rh + 0.5;
+    double q = fma(1.0 / 6227020800.0, rh, 1.0 / 479001600.0);  /* 1/13!, 1/12! */
+    q = fma(q, rh, 1.0 / 39916800.0);
+    q = fma(q, rh, 1.0 / 3628800.0);
+    q = fma(q, rh, 1.0 / 362880.0);
+    q = fma(q, rh, 1.0 / 40320.0);
+    q = fma(q, rh, 1.0 / 5040.0);
+    q = fma(q, rh, 1.0 / 720.0);
+    q = fma(q, rh, 1.0 / 120.0);
+    q = fma(q, rh, 1.0 / 24.0);
+    q = fma(q, rh, 1.0 / 6.0);
+    q = fma(q, rh, 0.5);
     q = q * (rh * rh);                                         /* e^r - 1 - r */
     double b;
     const double a = wm_two_sum(1.0, rh, &b);
@@ -182,9 +203,9 @@ WM_FN double wm_tanh(double x)
     const double nl = ne + el;
     const double dh = wm_two_sum(eh, 1.0, &de);    /* e^2a + 1 = dh + dl */
     const double dl = de + el;
-    const double t0 = nh / dh;                     /* one correction step of the quotient */
+    const double t0 = wm_div(nh, dh);              /* one correction step of the quotient */
     const double r = (fma(-t0, dh, nh) + nl) - t0 * dl;
-    const double t = t0 + r / dh;
+    const double t = t0 + wm_div(r, dh);
     return x < 0.0 ? -t : t;
 }
 
