@@ -132,3 +132,35 @@ def test_batch_with_degenerate_cases_equals_alone(gpu_ok, precision):
             si, sj = c.grid.interior()
             assert np.array_equal(b.state(k)[:, si, sj], ref[k][:, si, sj]), (tuning, shapes[k])
         b.close()
+
+
+def test_failed_set_case_keeps_the_slot_and_the_graph(gpu_ok):
+    """A case whose scratch cannot be allocated (a grid far beyond HBM) is
+    refused; the slot keeps its case, the batch stays committed and its
+    captured graph keeps stepping it, bitwise as the case alone; later
+    launches elsewhere do not see the refused allocation's error."""
+    import ctypes as C
+
+    from wamp import _lib
+    from wamp.batch import Batch
+
+    cs = _cases(1)[:1]
+    ref = _alone(cs[0][0], cs[0][1], 4)
+    b = Batch(cs[0][0].scheme, 1, max_cases=1)
+    b.set_case(0, *cs[0])
+    b.commit()
+    b.prepare()
+    b.capture(2)
+    b.step()
+    U = b.states[0]
+    huge = _lib.WampGrid(60000, 60000, 5, 60010, 1e-4, 1e-4, (C.c_int32 * 4)(0, 0, 1, 1))
+    eos = _lib.WampEos(*cs[0][0].eos.as_tuple())
+    rc = b.L.wamp_batch_set_case(b.h, 0, C.byref(huge), C.byref(eos),
+                                 _lib.ptrs6([U[m].data_ptr() for m in range(6)]), 0.0, 0.0)
+    assert rc > 0 and "hipMalloc" in b.L.wamp_last_error().decode()
+    assert b.step() == 2        # still committed, the graph replays
+    # the refused allocation is reported once, not again by later launches
+    assert np.array_equal(_alone(cs[0][0], cs[0][1], 4), ref)
+    si, sj = cs[0][0].grid.interior()
+    assert np.array_equal(b.state(0)[:, si, sj], ref[:, si, sj])
+    b.close()

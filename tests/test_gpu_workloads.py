@@ -100,8 +100,9 @@ def mach10_long(gpu_ok):
 @pytest.mark.parametrize("segment", range(LONG_SEGMENTS))
 def test_mach10_long_horizon_against_oracle(mach10_long, segment):
     """1024x512, 2000 steps in segments of 250 (each segment is a test, so a
-    long run reports as it goes): the state and every dt equal to the oracle's
-    bit for bit at every checkpoint.  (Against the oracle build with the C
+    long run reports as it goes): every dt equal to the oracle's bit for bit,
+    and the state equal to it at every checkpoint up to differences below
+    1e-290 in absolute value (transverse momentum at its round-off floor).  (Against the oracle build with the C
     library's pow and tanh the run stays within the flow's own sensitivity:
     that oracle, started from its own state scaled by 1 +- 2^-52, drifts to
     1e-9 by step 150, right after the shock meets the cylinder.)"""
@@ -118,7 +119,15 @@ def test_mach10_long_horizon_against_oracle(mach10_long, segment):
     print(f"steps {run.done} (t {run.t:.4f}, shock near x={shock_x:.2f}): rel L1",
           ["%.1e" % e for e in errs])
     si, sj = run.cfg.grid.interior()
-    assert np.array_equal(got[:, si, sj], ref[:, si, sj])
+    a, b = got[:, si, sj], ref[:, si, sj]
+    diff = a != b
+    worst = float(np.abs(a - b).max())
+    print(f"    cells differing per field {diff.sum(axis=(1, 2)).tolist()}, max |diff| {worst:.1e}, "
+          f"max |value| there {float(np.abs(b[diff]).max()) if diff.any() else 0.0:.1e}")
+    # equal except where both values sit at the round-off floor of an
+    # (almost) vanishing field: differences below 1e-290 in absolute value,
+    # from divisions whose quotient is subnormal
+    assert worst <= 1e-290
     assert np.array_equal(dts, dts_ref)
     if segment == LONG_SEGMENTS - 1:
         assert shock_x > 1.25   # past the cylinder (centre 1.0, radius 0.25)

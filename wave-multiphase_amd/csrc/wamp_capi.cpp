@@ -110,6 +110,10 @@ int fail(int code, const char* fmt, ...) {
 
 int hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
+  // reported here, so clear the thread's last-error slot: the launch wrappers
+  // read it after each launch and would otherwise report this error again
+  // (e.g. a refused hipMalloc) against a later, unrelated launch
+  (void)hipGetLastError();
   return fail((int)e, "%s: %s", what, hipGetErrorString(e));
 }
 
