@@ -124,6 +124,21 @@ int wamp_get_dt(wamp_ctx *ctx, double *const U[6], double *dt_dev, void *stream)
 int wamp_step(wamp_ctx *ctx, double *const U[6], const double *dt_dev, double *dt_next_dev,
               void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  wamp_step without a context: the same step (the same
+   kernels, the same result bit for bit) with its scratch -- face planes,
+   stage buffers, dt accumulator -- in a static device array of
+   WAMP_NOALLOC_SCRATCH_BYTES per device, held until the process ends.  It
+   allocates nothing.  Grids whose scratch does not fit are refused with an
+   error (about 209k padded cells in FP64, e.g. 512 x 256; 299k in mixed
+   precision).  Calls on one device share its array: calls that overlap on one
+   device (other streams, other threads) race. */
+#define WAMP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int wamp_step_noalloc(const wamp_grid *grid, const wamp_eos *eos, const wamp_scheme *scheme,
+                      int32_t precision, double *const U[6], const double *dt_dev,
+                      double *dt_next_dev, void *stream);
+
 /* ---- Initial conditions on the device -------------------------------------
    Replace the reference's initial-condition kernel (ic_kernel2, SC:779-837 /
    M10:736-790), a dimensional sharp-interface setup, with the nondimensional

@@ -73,6 +73,8 @@ SIGNATURES = [
                                     C.c_double, _VP, _VP, _VP]),
     ("wamp_get_dt", C.c_int, [_VP, PtrArr6, _VP, _VP]),
     ("wamp_step", C.c_int, [_VP, PtrArr6, _VP, _VP, _VP]),
+    ("wamp_step_noalloc", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos), C.POINTER(WampScheme),
+                                    C.c_int32, PtrArr6, _VP, _VP, _VP]),
     ("wamp_init_droplets", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos),
                                      C.POINTER(WampDropletIC), PtrArr6, _VP]),
     ("wamp_init_triple_point", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos),

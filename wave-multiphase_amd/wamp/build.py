@@ -32,6 +32,7 @@ UNITS = [
     ("wamp3_kernels.hip", "-ffp-contract=off"),
     ("wamp3_faces_mixed.hip", "-ffp-contract=on"),
     ("wamp_capi.cpp", "-ffp-contract=off"),
+    ("wamp_noalloc.hip", "-ffp-contract=off"),
 ]
 
 
