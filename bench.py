@@ -163,9 +163,10 @@ def threed_leg(n: int, warmup: int, steps: int):
 
 
 def _timed_batch(work, precision: int, steps: int):
-    """One batch of `work` in `precision`: autotuned, step graph-captured, one
-    warm-up replay from the initial states; returns (seconds for `steps` steps,
-    tuning, graph steps, the batch with its final states)."""
+    """One batch of `work` in `precision`.  Setup (case table, initial
+    sequence, autotuning, graph capture and one warm-up replay from the
+    initial states) and stepping are timed separately; returns (setup s,
+    stepping s, tuning, graph steps, the batch with its final states)."""
     import dataclasses
 
     import torch
@@ -173,6 +174,8 @@ def _timed_batch(work, precision: int, steps: int):
     from wamp.batch import Batch
     from wamp.config import SCHEME_MP5
 
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     b = Batch(SCHEME_MP5, precision, max_cases=len(work))
     for k, (cfg, U0) in enumerate(work):
         b.set_case(k, dataclasses.replace(cfg, precision=precision), U0)
@@ -188,19 +191,22 @@ def _timed_batch(work, precision: int, steps: int):
     b.prepare()
     del snap
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    t1 = time.perf_counter()
     done = 0
     while done < steps:
         done += b.step()
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, tuning, gsteps, b
+    return t1 - t0, time.perf_counter() - t1, tuning, gsteps, b
 
 
 def batched_leg(n_cases: int, steps: int, seed: int = 4):
-    """64 seeded uneven shock-droplet cases: plain FP64 one case at a time versus
-    one batch in mixed precision (autotuned, step graph-captured), and, to
-    separate batching from precision, the same batch in FP64.  All run `steps`
-    steps per case; cases/hour is normalised to 500-step cases."""
+    """64 seeded uneven shock-droplet cases, `steps` steps each: the plain FP64
+    solver one case at a time versus one batch in mixed precision (autotuned,
+    step graph-captured), and, to separate batching from precision, the same
+    batch in FP64.  Every mode is timed over the same boundary: the steps
+    alone (cases/hour), with the setup (plain: context, upload, initial
+    sequence, first dt; batched: case table, initial sequence, autotuning,
+    capture) timed separately and also reported included."""
     import numpy as np
     import torch
 
@@ -213,44 +219,57 @@ def batched_leg(n_cases: int, steps: int, seed: int = 4):
     cells = sum(c.grid.nx * c.grid.ny for c, _ in work)
     # plain FP64, one case after another (each case's state resident on the GPU)
     plain_states = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    t_setup = t_steps = 0.0
     for cfg, U0 in work:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         s = Solver(cfg)
         s.set_state(U0)
         s.prepare()
         s.compute_dt()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         for _ in range(steps):
             s.step_device()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_setup += t1 - t0
+        t_steps += t2 - t1
         if len(plain_states) < 4:
             plain_states.append(s.state())
         s.close()
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    t_b64, _, _, b = _timed_batch(work, PREC_FP64, steps)
+    setup64, t_b64, _, _, b = _timed_batch(work, PREC_FP64, steps)
     same64 = all(np.array_equal(b.state(k)[:, cfg.grid.interior()[0], cfg.grid.interior()[1]],
                                 plain_states[k][:, cfg.grid.interior()[0], cfg.grid.interior()[1]])
                  for k, (cfg, _) in enumerate(work[:len(plain_states)]))
     b.close()
-    t_batch, tuning, gsteps, b = _timed_batch(work, PREC_MIXED, steps)
+    setup_mx, t_batch, tuning, gsteps, b = _timed_batch(work, PREC_MIXED, steps)
     spot = []
     for k in range(min(4, n_cases)):
         cfg = work[k][0]
         spot.append(accuracy.compare(cfg, b.state(k), plain_states[k]))
     b.close()
-    scale = 500.0 / steps
+    cph = lambda t: round(n_cases / t * 3600.0, 2)
     return {
         "workload": f"{n_cases} shock-droplet cases, grids from "
                     f"{sorted(set((c.grid.nx, c.grid.ny) for c, _ in work))}, seed {seed}",
         "cells": cells,
         "steps_per_case": steps,
-        "plain_fp64_cases_per_hour": round(n_cases / (t_plain * scale) * 3600.0, 2),
-        "batched_mixed_cases_per_hour": round(n_cases / (t_batch * scale) * 3600.0, 2),
-        "speedup": round(t_plain / t_batch, 3),
-        "plain_fp64_gcell_updates_per_s": round(cells * steps / t_plain / 1e9, 4),
+        "timing": "steps only (setup timed separately, and included in *_incl_setup)",
+        "plain_fp64_cases_per_hour": cph(t_steps),
+        "batched_mixed_cases_per_hour": cph(t_batch),
+        "speedup": round(t_steps / t_batch, 3),
+        "batched_fp64_cases_per_hour": cph(t_b64),
+        "batching_speedup_fp64": round(t_steps / t_b64, 3),
+        "plain_fp64_gcell_updates_per_s": round(cells * steps / t_steps / 1e9, 4),
         "batched_mixed_gcell_updates_per_s": round(cells * steps / t_batch / 1e9, 4),
-        "batched_fp64_cases_per_hour": round(n_cases / (t_b64 * scale) * 3600.0, 2),
-        "batching_speedup_fp64": round(t_plain / t_b64, 3),
+        "batched_fp64_gcell_updates_per_s": round(cells * steps / t_b64 / 1e9, 4),
+        "setup_s": {"plain_fp64": round(t_setup, 3), "batched_fp64": round(setup64, 3),
+                    "batched_mixed": round(setup_mx, 3)},
+        "cases_per_hour_incl_setup": {"plain_fp64": cph(t_steps + t_setup),
+                                      "batched_fp64": cph(t_b64 + setup64),
+                                      "batched_mixed": cph(t_batch + setup_mx)},
+        "speedup_incl_setup": round((t_steps + t_setup) / (t_batch + setup_mx), 3),
         "batched_fp64_equals_plain_bitwise": bool(same64),
         "tuning": list(tuning.as_tuple()),
         "graph_steps": gsteps,
@@ -283,7 +302,8 @@ def serving_leg(n: int, seed: int = 5, load: float = 0.7):
     _, plain = serve.serve_plain(serve.with_arrivals(reqs, None, precision=PREC_FP64),
                                  keep=lambda rid, U: plain_states.__setitem__(rid, U), log=_progress)
     _progress(f"serving: plain fp64 {n} requests {plain['elapsed_s']:.1f} s")
-    res_sat, sat = serve.serve_batched(serve.with_arrivals(reqs, None), log=_progress)
+    res_sat, sat = serve.serve_batched(serve.with_arrivals(reqs, None), log=_progress,
+                                       track_busy=True)
     _progress(f"serving: batched saturated {sat['elapsed_s']:.1f} s")
     rate = load * sat["completed"] / sat["elapsed_s"]
     res_poi, poi = serve.serve_batched(serve.with_arrivals(reqs, rate), log=_progress)
@@ -416,6 +436,42 @@ def valu_fracs(workload: str, ms_per_step: float):
             "source": f"profiles/step_fp64_work_{workload}.txt"}
 
 
+def late_window(s, done: int, start: int, steps: int) -> dict:
+    """The headline solver, already `done` steps in, advanced untimed to step
+    `start` (the shock in the water cylinder: it reaches the cylinder's front
+    near step 680 and leaves its back near step 3500 at 4096x2048), then
+    `steps` steps timed the same way as the headline window."""
+    import ctypes as C
+
+    import torch
+
+    from wamp import _lib
+
+    for _ in range(max(0, start - done)):
+        s.step_device()
+    torch.cuda.synchronize()
+    _lib.check(s.L.wamp_timing_enable(s.ctx, steps), "wamp_timing_enable")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        s.step_device()
+    e1.record()
+    torch.cuda.synchronize()
+    t_ms = e0.elapsed_time(e1)
+    ms = (C.c_double * 2)()
+    nl = (C.c_int64 * 2)()
+    _lib.check(s.L.wamp_timing_read(s.ctx, ms, nl), "wamp_timing_read")
+    _lib.check(s.L.wamp_timing_enable(s.ctx, 0), "wamp_timing_enable")
+    g = s.cfg.grid
+    return {"first_step": max(start, done), "steps": steps,
+            "value": round(g.nx * g.ny * steps / (t_ms / 1e3) / 1e9, 6),
+            "ms_per_step": round(t_ms / steps, 4),
+            "kernels_ms": {"k_faces": round(ms[0] / max(nl[0], 1), 4),
+                           "k_limit_update": round(ms[1] / max(nl[1], 1), 4)},
+            "state_finite": bool(torch.isfinite(s.U).all().item())}
+
+
 def _progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -433,13 +489,16 @@ def main(argv=None):
                     help="steps of the FP32-mixed leg on the same case (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=64,
                     help="cases of the batched leg (0 = skip)")
-    ap.add_argument("--batch-steps", type=int, default=100, help="steps per case in the batched leg")
+    ap.add_argument("--batch-steps", type=int, default=500, help="steps per case in the batched leg")
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--threed-n", type=int, default=160,
                     help="the 3D leg's shock-sphere case has 2n x n x n cells")
     ap.add_argument("--threed-steps", type=int, default=20,
                     help="timed steps of the 3D leg (0 = skip)")
+    ap.add_argument("--late-start", type=int, default=3500,
+                    help="also time --steps steps of the headline case from this step on, after "
+                         "the shock has crossed the cylinder (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="before the run, copy a MIB-MiB device buffer once (a plain copy kernel "
                          "of known size, to scale hardware traffic counters)")
@@ -509,6 +568,7 @@ def main(argv=None):
     alg_bytes = ALG_BYTES_PER_CELL_UPDATE / 3.0 * cells
     achieved = alg_bytes / (faces_ms / 1e3) / 1e9
     step_gbs = ALG_BYTES_PER_CELL_UPDATE * cells * K / (t_ms / 1e3) / 1e9
+    kvalu = kernel_valu(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces", faces_ms)
 
     out = {
         "metric": "Gcell-updates/s",
@@ -540,7 +600,13 @@ def main(argv=None):
             "traffic": measured_traffic(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces"),
             "launch_ms": round(faces_ms, 4),
             "launches": int(nl[0]),
-            **kernel_valu(f"mach10_cylinder_{g.nx}x{g.ny}", "k_faces", faces_ms),
+            **kvalu,
+            # the bound that binds: the face kernel is limited by FP64 VALU
+            # issue, not HBM (the HBM fraction above is the algorithmic-bytes
+            # framing); valu_issue_frac = issued VALU wave instructions x 4
+            # cycles / (1024 SIMDs x 2.4 GHz x launch time)
+            "binding": {"bound": "valu_issue", "frac": kvalu.get("valu_issue_frac"),
+                        "fp64_flop_frac": kvalu.get("fp64_frac")},
         },
         "kernels_ms": {"k_faces": round(faces_ms, 4), "k_limit_update": round(lu_ms, 4)},
         "limiter": limiter_rates(f"mach10_cylinder_{g.nx}x{g.ny}", lu_ms, copy_gbs),
@@ -550,6 +616,10 @@ def main(argv=None):
         "state_finite": finite,
         "dt_last": dt_now,
     }
+    if args.late_start > 0:
+        out["late_window"] = late_window(s, args.warmup + K, args.late_start, K)
+        _progress(f"late window from step {out['late_window']['first_step']}: "
+                  f"{out['late_window']['value']:.4f} Gcell-updates/s")
     if args.mixed_steps > 0:
         out["mixed"] = mixed_leg(cfg, U0, args.warmup, args.mixed_steps)
     s.close()

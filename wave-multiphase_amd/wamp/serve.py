@@ -73,7 +73,7 @@ class Result:
 class Server:
     def __init__(self, scheme=SCHEME_MP5, precision: int = PREC_MIXED, max_slots: int = 64,
                  hbm_budget: float = 64e9, round_steps: int = 10, tuning: Tuning | None = None,
-                 clock=time.perf_counter, sleep=time.sleep, batch=None):
+                 clock=time.perf_counter, sleep=time.sleep, batch=None, track_busy: bool = False):
         # `batch`: a Batch-like object to drive (the host logic's tests pass a stand-in)
         self.batch = batch if batch is not None else Batch(scheme, precision, max_slots)
         self.scheme, self.precision = scheme, precision
@@ -90,6 +90,10 @@ class Server:
         self.queue: list[Request] = []
         self.results: list[Result] = []
         self.rounds = 0
+        # device-busy accounting: an event pair around every round's replay
+        self.track_busy = track_busy
+        self._events = []
+        self.busy_s = None
         if tuning:
             self.batch.set_tuning(tuning)
 
@@ -132,7 +136,8 @@ class Server:
         return new
 
     def run(self, until_empty: bool = True, max_rounds: int = 10 ** 9, log=None):
-        """Serve until every submitted request has finished."""
+        """Serve until every submitted request has finished.  With track_busy,
+        busy_s is the device time of the rounds' replays (event pairs)."""
         t_start = self.clock()
         while self.rounds < max_rounds:
             now = self.clock()
@@ -150,12 +155,28 @@ class Server:
                 if wait > 0:
                     self.sleep(min(wait, 0.05))
                 continue
-            self.batch.step()
+            if self.track_busy:
+                import torch
+
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self.batch.step()
+                e1.record()
+                self._events.append((e0, e1))
+            else:
+                self.batch.step()
             self.rounds += 1
             if log and self.rounds % 200 == 0:
                 log(f"  round {self.rounds}: resident {len(self.slot_req)}, queued {len(self.queue)}, "
                     f"done {len(self.results)}, HBM {self.used / 1e9:.2f} GB")
-        return self.clock() - t_start
+        elapsed = self.clock() - t_start
+        if self.track_busy:
+            import torch
+
+            torch.cuda.synchronize()
+            self.busy_s = sum(a.elapsed_time(b) for a, b in self._events) / 1e3
+            self._events = []
+        return elapsed
 
     def close(self):
         self.batch.close()
@@ -269,10 +290,11 @@ def serve_plain(reqs, clock=time.perf_counter, keep=None, log=None):
 
 def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int = 64,
                   hbm_budget: float = 64e9, round_steps: int = 10, tuning: Tuning | None = None,
-                  log=None):
+                  log=None, track_busy: bool = False):
     """Run `reqs` (arrivals as offsets from the start) through a Server."""
     srv = Server(scheme=scheme, precision=precision, max_slots=max_slots, hbm_budget=hbm_budget,
-                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 26))
+                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 26),
+                 track_busy=track_busy)
     base = srv.clock()
     for r in reqs:
         srv.submit(dataclasses.replace(r, arrival=base + r.arrival))
@@ -284,6 +306,8 @@ def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int 
     st = _stats(srv.results, elapsed)
     st.update(rounds=srv.rounds, peak_resident=srv.peak_resident,
               peak_hbm_gb=round(srv.peak_used / 1e9, 3))
+    if track_busy:
+        st.update(device_busy_s=srv.busy_s, device_busy_frac=srv.busy_s / elapsed)
     results = srv.results
     srv.close()
     return results, st
