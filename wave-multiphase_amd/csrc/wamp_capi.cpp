@@ -558,7 +558,7 @@ struct wamp_batch {
   // tuning
   int tile;            // 0: 8x32 face tiles, 1: 16x16
   int faces_bpcu;      // persistent face blocks per CU
-  int lu_bpcu;         // persistent limiter blocks (4 waves) per CU
+  int lu_bpcu;         // persistent limiter blocks (4 waves) per CU; 0: one wave per item
   int lu_seg;          // limiter rows per wave
   // host mirror of the case table
   std::vector<CaseDesc> desc;
@@ -621,7 +621,7 @@ int wamp_batch_create(const wamp_scheme* scheme, int32_t precision, int32_t max_
   b->t1 = wm_tanh(1.8 * 0.5);
   b->tile = 0;
   b->faces_bpcu = 3;
-  b->lu_bpcu = 2;
+  b->lu_bpcu = 0;
   b->lu_seg = 16;
   b->desc.assign(max_cases, CaseDesc());
   b->used.assign(max_cases, 0);
@@ -734,8 +734,8 @@ int wamp_batch_set_tuning(wamp_batch* b, int32_t tile, int32_t faces_bpcu, int32
                           int32_t lu_seg) {
   if (!b) return fail(-1, "NULL batch");
   if (tile < 0 || tile > 1) return fail(-3, "tile must be 0 (8x32) or 1 (16x16)");
-  if (faces_bpcu < 1 || faces_bpcu > 16 || lu_bpcu < 1 || lu_bpcu > 16)
-    return fail(-3, "blocks per CU must be in [1, 16]");
+  if (faces_bpcu < 1 || faces_bpcu > 16) return fail(-3, "face blocks per CU must be in [1, 16]");
+  if (lu_bpcu < 0 || lu_bpcu > 16) return fail(-3, "limiter blocks per CU must be in [0, 16]");
   if (lu_seg != 8 && lu_seg != 16 && lu_seg != 20 && lu_seg != 26 && lu_seg != 32)
     return fail(-3, "lu_seg must be 8, 16, 20, 26 or 32");
   b->tile = tile;
@@ -820,7 +820,9 @@ int wamp_batch_set_done(wamp_batch* b, int32_t slot, int32_t done, void* stream)
 static int batch_enqueue_step(wamp_batch* b, hipStream_t st) {
   const int n = b->max_cases;
   const int fb = std::max(1, std::min(b->total_tiles, b->faces_bpcu * b->cu_count));
-  const int lb = std::max(1, (b->total_items + 3) / 4);  // one wave per limiter item
+  // limiter: one wave per item (lu_bpcu 0), or lu_bpcu persistent blocks per CU
+  const int per_item = std::max(1, (b->total_items + 3) / 4);
+  const int lb = b->lu_bpcu == 0 ? per_item : std::min(per_item, b->lu_bpcu * b->cu_count);
   for (int s = 0; s < 3; ++s) {
     if (int rc = hip_check(wamp_launch_faces_batch(b->desc_dev, b->off_dev, n, &b->sc, s,
                                                    b->scheme.base, b->scheme.shear,

@@ -852,25 +852,29 @@ k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, in
   // from it: case, descriptor, rows) in scalar registers
   __shared__ double carry[LU_WAVES * LU_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);
-  const int it = blockIdx.x * (blockDim.x / MW) + wave;
-  if (it >= item_off[ncase]) return;
-  const int k = find_case(item_off, ncase, it);
-  if (sc.done[k]) return;
-  const CaseDesc& c = cs[k];
-  const int li = it - item_off[k];
+  const int total = item_off[ncase];
   const double alpha = stage == 0 ? 0.0 : stage == 1 ? 0.75 : 1.0 / 3.0;
   const double beta = stage == 0 ? 1.0 : stage == 1 ? 0.25 : 2.0 / 3.0;
-  State Un, U, Uo;
+  // items wave, wave + waves in the grid, ...: one item per wave when the
+  // grid has a wave per item, a persistent stride otherwise
+  const int stride = (int)gridDim.x * (blockDim.x / MW);
+  for (int it = blockIdx.x * (blockDim.x / MW) + wave; it < total; it += stride) {
+    const int k = find_case(item_off, ncase, it);
+    if (sc.done[k]) continue;
+    const CaseDesc& c = cs[k];
+    const int li = it - item_off[k];
+    State Un, U, Uo;
 #pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    Un.f[m] = c.u[0][m];
-    U.f[m] = c.u[stage][m];
-    Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
+    for (int m = 0; m < 6; ++m) {
+      Un.f[m] = c.u[0][m];
+      U.f[m] = c.u[stage][m];
+      Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
+    }
+    double v = lu_wave<SEG, F32>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair,
+                                 stage == 2, li % c.strips, li / c.strips,
+                                 carry + wave * (LU_CARRY * MW));
+    if (stage == 2) wave_min_atomic(v, sc.acc + k);
   }
-  double v = lu_wave<SEG, F32>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair,
-                               stage == 2, li % c.strips, li / c.strips,
-                               carry + wave * (LU_CARRY * MW));
-  if (stage == 2) wave_min_atomic(v, sc.acc + k);
 }
 
 // After a step: t += dt; next dt = cfl * min, clipped to the end time

@@ -28,14 +28,14 @@ def _torch():
 class Tuning:
     tile: int = 0          # 0: 8x32 face tiles, 1: 16x16
     faces_bpcu: int = 3    # persistent face blocks per CU
-    lu_bpcu: int = 2       # persistent limiter blocks (4 waves) per CU
+    lu_bpcu: int = 0       # persistent limiter blocks (4 waves) per CU; 0: one wave per item
     lu_seg: int = 16       # limiter rows per wave
 
     def as_tuple(self):
         return (self.tile, self.faces_bpcu, self.lu_bpcu, self.lu_seg)
 
 
-TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (2, 4) for s in (8, 16, 26)]
+TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (0, 2) for s in (8, 16, 26)]
 
 
 class Batch:
@@ -58,6 +58,7 @@ class Batch:
         self.h = h
         self.cfgs: dict[int, CaseConfig] = {}
         self.states: dict[int, object] = {}
+        self.t0: dict[int, float] = {}
         self.tuning = Tuning()
         self.graph_steps = 0
 
@@ -99,11 +100,13 @@ class Batch:
                                               float(t0), float(t_end)), "wamp_batch_set_case")
         self.cfgs[slot] = cfg
         self.states[slot] = U
+        self.t0[slot] = float(t0)
 
     def clear_case(self, slot: int):
         _lib.check(self.L.wamp_batch_clear_case(self.h, int(slot)), "wamp_batch_clear_case")
         self.cfgs.pop(slot, None)
         self.states.pop(slot, None)
+        self.t0.pop(slot, None)
 
     def state(self, slot: int) -> np.ndarray:
         return self.states[slot].cpu().numpy().copy()
@@ -158,10 +161,15 @@ class Batch:
         """Time `steps` steps per candidate on a snapshot of the states; keep the fastest.
 
         The work decomposition does not change any result, so the snapshot is
-        only restored to leave the cases where they were."""
+        only restored to leave the cases where they were.  The restore also
+        resets each case's time to its start time (prepare), so autotune runs
+        before the batch steps, and refuses otherwise."""
         torch = _torch()
-        snap = {k: v.clone() for k, v in self.states.items()}
         t_before, _, _ = self.status()
+        moved = [k for k in self.states if t_before[k] != self.t0[k]]
+        if moved:
+            raise RuntimeError(f"autotune runs before the batch steps (slots {moved} have advanced)")
+        snap = {k: v.clone() for k, v in self.states.items()}
         best, best_ms = None, float("inf")
         for cand in (space or TUNING_SPACE):
             self.set_tuning(cand)
