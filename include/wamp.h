@@ -139,6 +139,40 @@ int wamp_step_noalloc(const wamp_grid *grid, const wamp_eos *eos, const wamp_sch
                       int32_t precision, double *const U[6], const double *dt_dev,
                       double *dt_next_dev, void *stream);
 
+/* ---- The CPU device ------------------------------------------------------
+   The reference's --cpu (SC:15-21, M10:14-20: wp.init() with device "cpu")
+   runs its Warp kernels on the host.  These entry points run this library's
+   FP64 step on the host: the same per-cell code as the gfx950 kernels (the
+   face tile phases, the limiter stages and the update are host-and-device
+   functions), with the IEEE divide and square root that the kernels' division
+   and square-root sequences reproduce, and the same pow and tanh -- the
+   result is the FP64 GPU step's, bit for bit.  Pointers are HOST pointers,
+   with the layout of the device entry points; every call returns when done.
+   FP64 only (precision is not an argument).  One context is not re-entrant:
+   calls on one context from several threads race.  No HIP call is made. */
+typedef struct wamp_cpu_ctx wamp_cpu_ctx;
+
+/* threads: worker threads per call; 0: OMP_NUM_THREADS when set, else the
+   CPUs of the process's affinity mask.  Allocates
+   the scratch of the device context in host memory (40 planes) plus three
+   limiter theta planes. */
+int wamp_cpu_ctx_create(const wamp_grid *grid, const wamp_eos *eos, const wamp_scheme *scheme,
+                        int32_t threads, wamp_cpu_ctx **out);
+int wamp_cpu_ctx_destroy(wamp_cpu_ctx *ctx);
+/* Worker threads of a context. */
+int32_t wamp_cpu_threads(const wamp_cpu_ctx *ctx);
+/* As wamp_prepare, wamp_apply_bc, wamp_cons_to_prim, wamp_compute_fluxes
+   (SC:1998-2001, 1899-1905, 1913-1917, 1919-1947). */
+int wamp_cpu_prepare(wamp_cpu_ctx *ctx, double *const U[6]);
+int wamp_cpu_apply_bc(wamp_cpu_ctx *ctx, double *const U[6], double *const P[3]);
+int wamp_cpu_cons_to_prim(wamp_cpu_ctx *ctx, double *const U[6], double *const P[3]);
+int wamp_cpu_compute_fluxes(wamp_cpu_ctx *ctx, double *const U[6], double *const F[28]);
+/* get_dt, SC:1986-1995: *dt = cfl * min over interior cells. */
+int wamp_cpu_get_dt(wamp_cpu_ctx *ctx, double *const U[6], double *dt);
+/* One SSP-RK3 step in place on U with time step dt (SC:2028-2059); the next
+   dt, cfl * min of the new state, to *dt_next (may be NULL). */
+int wamp_cpu_step(wamp_cpu_ctx *ctx, double *const U[6], double dt, double *dt_next);
+
 /* ---- Initial conditions on the device -------------------------------------
    Replace the reference's initial-condition kernel (ic_kernel2, SC:779-837 /
    M10:736-790), a dimensional sharp-interface setup, with the nondimensional

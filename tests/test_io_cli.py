@@ -53,9 +53,6 @@ def test_restart_rejects_bad_files(tmp_path):
         io.load_restart(str(bad), cfg)
 
 
-def test_cli_cpu_flag_is_refused():
-    r = subprocess.run([sys.executable, CLI, "--cpu"], capture_output=True, text=True)
-    assert r.returncode == 2 and "GPU only" in r.stderr
 
 
 @pytest.mark.gpu

@@ -17,10 +17,18 @@
 //    formed once per cell by the callers, not once per face.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "wamp_math.h"
 #include "wamp_types.h"
+
+// The scalar helpers are host-and-device functions: the FP64 step also runs
+// on the host (wamp_cpu.hip, the CPU device) from this same source.  Device
+// builds use the hardware reciprocal / square-root sequences below, host
+// builds the IEEE operations those sequences reproduce bit for bit, so a
+// host step gives the device step's bits.
+#define WAMP_FN __host__ __device__ __forceinline__
 
 // The FP64 kernels (wamp_kernels.hip) and the FP32-mixed face kernels
 // (wamp_faces_mixed.hip, defining WAMP_MIXED_TU) are separate translation
@@ -38,22 +46,22 @@
 WAMP_DEV_BEGIN
 
 // Warp builtin semantics: comparison selects, sign(0) = +1.
-template <typename T> __device__ __forceinline__ T wmin(T a, T b) { return a < b ? a : b; }
-template <typename T> __device__ __forceinline__ T wmax(T a, T b) { return a > b ? a : b; }
-template <typename T> __device__ __forceinline__ T wsign(T x) { return x < T(0) ? T(-1) : T(1); }
-template <typename T> __device__ __forceinline__ bool isnan_(T x) { return __builtin_isnan(x); }
+template <typename T> WAMP_FN T wmin(T a, T b) { return a < b ? a : b; }
+template <typename T> WAMP_FN T wmax(T a, T b) { return a > b ? a : b; }
+template <typename T> WAMP_FN T wsign(T x) { return x < T(0) ? T(-1) : T(1); }
+template <typename T> WAMP_FN bool isnan_(T x) { return __builtin_isnan(x); }
 // FP32 (the mixed mode's face arithmetic only): the hardware v_min_f32 /
 // v_max_f32, one instruction instead of a compare and a select.  For
 // operands that are not NaN they return the same value as the comparison
 // selects (a pair of zeros may come back with the other sign).
-template <> __device__ __forceinline__ float wmin<float>(float a, float b) { return __builtin_fminf(a, b); }
-template <> __device__ __forceinline__ float wmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+template <> WAMP_FN float wmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> WAMP_FN float wmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
 // sign inside the minmod limiters: there every term is multiplied by the
 // smallest magnitude of its arguments, which is zero whenever an argument is
 // a zero of either sign, so copysign (one v_bfi) gives the same value as
 // wsign for FP32.
-template <typename T> __device__ __forceinline__ T msign(T x) { return wsign(x); }
-template <> __device__ __forceinline__ float msign<float>(float x) { return __builtin_copysignf(1.0f, x); }
+template <typename T> WAMP_FN T msign(T x) { return wsign(x); }
+template <> WAMP_FN float msign<float>(float x) { return __builtin_copysignf(1.0f, x); }
 
 // Division, square root and tanh by arithmetic type.  The FP64 kernels keep
 // the correctly rounded results (the parity path), through the shortened
@@ -69,7 +77,7 @@ template <> __device__ __forceinline__ float msign<float>(float x) { return __bu
 // correctly rounded quotient for finite a, b with no overflow or underflow,
 // i.e. bitwise a / b.  Three instructions where y is known (a host constant
 // or a reciprocal shared by several numerators).
-__device__ __forceinline__ double quotient(double a, double y, double b) {
+WAMP_FN double quotient(double a, double y, double b) {
   const double q = a * y;
   const double r = fma(-b, q, a);
   return fma(r, y, q);
@@ -78,12 +86,16 @@ __device__ __forceinline__ double quotient(double a, double y, double b) {
 // 1/b from v_rcp_f64 and two Newton steps: the reciprocal steps of the
 // compiler's IEEE divide sequence (v_div_scale, v_rcp, 4 x v_fma, v_mul,
 // v_fma, v_div_fmas, v_div_fixup) without its operand scaling.
-__device__ __forceinline__ double recip_nr(double b) {
+WAMP_FN double recip_nr(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
   double y = __builtin_amdgcn_rcp(b);
   double e = fma(-b, y, 1.0);
   y = fma(y, e, y);
   e = fma(-b, y, 1.0);
   return fma(y, e, y);
+#else
+  return 1.0 / b;
+#endif
 }
 
 // FP64 division.
@@ -102,8 +114,10 @@ __device__ __forceinline__ double recip_nr(double b) {
 //  * The mixed unit's remaining FP64 arithmetic (pressure recovery,
 //    indicators, face positivity limiter, energy flux): one Newton step and
 //    the correction, within one ulp of the quotient, six instructions.
-__device__ __forceinline__ double ddiv(double a, double b) {
-#ifdef WAMP_MIXED_TU
+WAMP_FN double ddiv(double a, double b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+  return a / b;  // host: the IEEE quotient
+#elif defined(WAMP_MIXED_TU)
   double y = __builtin_amdgcn_rcp(b);
   const double e = fma(-b, y, 1.0);
   y = fma(y, e, y);
@@ -114,8 +128,14 @@ __device__ __forceinline__ double ddiv(double a, double b) {
   return quotient(a, recip_nr(b), b);
 #endif
 }
-__device__ __forceinline__ double dv(double a, double b) { return ddiv(a, b); }
-__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+WAMP_FN double dv(double a, double b) { return ddiv(a, b); }
+WAMP_FN float dv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a / b;  // FP32 arithmetic runs on the device only
+#endif
+}
 // FP64 square root of the sound speed (argument clamped to >= TINY = 1e-20).
 // FP64 kernels: the compiler's IEEE sqrt sequence (v_rsq_f64, two products,
 // seven FMAs) without its scaling of arguments below 2^-767 and its select
@@ -123,8 +143,8 @@ __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amd
 // sqrt(x) for finite x >= 2^-767; NaN stays NaN (+inf would give NaN, not
 // +inf, but the argument is finite in any admissible state).  The mixed unit
 // keeps the library sqrt for its few FP64 square roots.
-__device__ __forceinline__ double sq(double x) {
-#ifdef WAMP_MIXED_TU
+WAMP_FN double sq(double x) {
+#if !defined(__HIP_DEVICE_COMPILE__) || defined(WAMP_MIXED_TU)
   return sqrt(x);
 #else
   const double g = __builtin_amdgcn_rsq(x);
@@ -139,7 +159,13 @@ __device__ __forceinline__ double sq(double x) {
   return fma(d, h, s);
 #endif
 }
-__device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+WAMP_FN float sq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
 // FP64 tanh (THINC) out of line: the face kernel has up to 12 THINC sites;
 // one shared body keeps the FP64 face kernel's code within the 64 KB
 // instruction cache of a CU pair.  It runs only where THINC fires.  The FP64
@@ -147,20 +173,24 @@ __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x);
 // the same source give the same bits); the mixed unit keeps the library tanh
 // for its few FP64 THINC values.
 #ifdef WAMP_MIXED_TU
-__device__ __noinline__ double th(double x) { return tanh(x); }
+inline __host__ __device__ __noinline__ double th(double x) { return tanh(x); }
 #else
-__device__ __noinline__ double th(double x) { return wm_tanh(x); }
+inline __host__ __device__ __noinline__ double th(double x) { return wm_tanh(x); }
 #endif
-__device__ __forceinline__ float th(float x) {
+WAMP_FN float th(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   float t = __expf(-2.0f * fabsf(x));
   float r = (1.0f - t) * __builtin_amdgcn_rcpf(1.0f + t);
   return x < 0.0f ? -r : r;
+#else
+  return tanhf(x);
+#endif
 }
 
 // Face plane m at cell offset c in either storage of Faces.  The limiter
 // kernels take the storage as a template argument (F32: mixed storage).
 template <bool F32>
-__device__ __forceinline__ double fload(const Faces& F, int m, size_t c) {
+WAMP_FN double fload(const Faces& F, int m, size_t c) {
   if constexpr (F32) {
     if (face_is_energy(m)) return F.b[(long long)(m / 7) * F.s + (long long)c];
     return (double)F.h[(long long)face_slot32(m) * F.hs + (long long)c];
@@ -172,7 +202,7 @@ __device__ __forceinline__ double fload(const Faces& F, int m, size_t c) {
 // mixed ones write whichever storage F describes (the split entry point
 // wamp_compute_fluxes hands them 28 FP64 planes).
 template <typename T>
-__device__ __forceinline__ void fstore(const Faces& F, int m, size_t c, double v) {
+WAMP_FN void fstore(const Faces& F, int m, size_t c, double v) {
   if constexpr (sizeof(T) == sizeof(float)) {
     if (F.h) {
       if (face_is_energy(m)) F.b[(long long)(m / 7) * F.s + (long long)c] = v;
@@ -204,7 +234,7 @@ constexpr double AL_CLIP = 1.0e-8;  // cons_to_prim clamp, SC:852-853
 // ---------------------------------------------------------------- EOS ------
 // mix_gamma_pibig SC:83-88
 template <typename T>
-__device__ __forceinline__ void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
+WAMP_FN void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
   T a2 = T(1) - a1;
   gb = dv(a1, T(e.g1) - T(1)) + dv(a2, T(e.g2) - T(1));
   pb = dv(a1 * T(e.g1) * T(e.p1), T(e.g1) - T(1)) + dv(a2 * T(e.g2) * T(e.p2), T(e.g2) - T(1));
@@ -212,7 +242,7 @@ __device__ __forceinline__ void mix_gb_pb(const Eos& e, T a1, T& gb, T& pb) {
 // FP64: the four divisions by the constants g-1 use the host reciprocals
 // (bitwise the same quotients; gamma > 1 is checked at context creation).
 template <>
-__device__ __forceinline__ void mix_gb_pb<double>(const Eos& e, double a1, double& gb, double& pb) {
+WAMP_FN void mix_gb_pb<double>(const Eos& e, double a1, double& gb, double& pb) {
   const double a2 = 1.0 - a1;
   const double d1 = e.g1 - 1.0, d2 = e.g2 - 1.0;
   gb = quotient(a1, e.r1, d1) + quotient(a2, e.r2, d2);
@@ -220,7 +250,7 @@ __device__ __forceinline__ void mix_gb_pb<double>(const Eos& e, double a1, doubl
 }
 // mix_gamma_pinf SC:90-95
 template <typename T>
-__device__ __forceinline__ void mix_g_pi(const Eos& e, T a1, T& g, T& pi) {
+WAMP_FN void mix_g_pi(const Eos& e, T a1, T& g, T& pi) {
   T gb, pb;
   mix_gb_pb(e, a1, gb, pb);
   g = dv(T(1), gb) + T(1);
@@ -228,7 +258,7 @@ __device__ __forceinline__ void mix_g_pi(const Eos& e, T a1, T& g, T& pi) {
 }
 // sound_speed SC:97-103
 template <typename T>
-__device__ __forceinline__ T sound_speed(const Eos& e, T rho, T p, T a1) {
+WAMP_FN T sound_speed(const Eos& e, T rho, T p, T a1) {
   T g, pi;
   mix_g_pi(e, a1, g, pi);
   T val = dv(g * (p + pi), rho);
@@ -237,7 +267,7 @@ __device__ __forceinline__ T sound_speed(const Eos& e, T rho, T p, T a1) {
 }
 // prim_to_energy SC:105-110
 template <typename T>
-__device__ __forceinline__ T prim_energy(const Eos& e, T ar1, T ar2, T u, T v, T p, T a1) {
+WAMP_FN T prim_energy(const Eos& e, T ar1, T ar2, T u, T v, T p, T a1) {
   T rho = ar1 + ar2;
   T gb, pb;
   mix_gb_pb(e, a1, gb, pb);
@@ -245,7 +275,7 @@ __device__ __forceinline__ T prim_energy(const Eos& e, T ar1, T ar2, T u, T v, T
 }
 // clamp01_pp SC:112-121
 template <typename T>
-__device__ __forceinline__ T clamp01_pp(T a1) {
+WAMP_FN T clamp01_pp(T a1) {
   T a = a1;
   if (isnan_(a)) a = T(PP_ALPHA);
   if (a < T(PP_ALPHA)) a = T(PP_ALPHA);
@@ -254,7 +284,7 @@ __device__ __forceinline__ T clamp01_pp(T a1) {
 }
 // pressure recovery from the conserved state, cons_to_prim_kernel SC:856-860
 // (always FP64: p is the small difference of E and the p_inf term).
-__device__ __forceinline__ double pressure(const Eos& e, double ar1, double ar2, double ru,
+WAMP_FN double pressure(const Eos& e, double ar1, double ar2, double ru,
                                            double rv, double E, double a1, double& uu,
                                            double& vv) {
   double rho = ar1 + ar2;
@@ -267,7 +297,7 @@ __device__ __forceinline__ double pressure(const Eos& e, double ar1, double ar2,
 
 // ------------------------------------------------------- positivity --------
 // pp_theta_lower SC:123-138
-__device__ __forceinline__ double theta_lower(double base, double high, double floor_) {
+WAMP_FN double theta_lower(double base, double high, double floor_) {
   double theta = 1.0;
   if (isnan_(base) || isnan_(high)) {
     theta = 0.0;
@@ -280,7 +310,7 @@ __device__ __forceinline__ double theta_lower(double base, double high, double f
   return theta;
 }
 // rho_ctilde2_cons SC:242-258
-__device__ __forceinline__ double rho_ct2_cons(const Eos& e, double m1, double m2, double mx,
+WAMP_FN double rho_ct2_cons(const Eos& e, double m1, double m2, double mx,
                                                double my, double E, double a1_in) {
   double a1 = clamp01_pp(a1_in);
   if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
@@ -295,7 +325,7 @@ __device__ __forceinline__ double rho_ct2_cons(const Eos& e, double m1, double m
 }
 // theta_cons_stage SC:268-292, specialised per stage
 template <int STAGE>
-__device__ __forceinline__ double theta_stage(const Eos& e, const double* l, const double* h) {
+WAMP_FN double theta_stage(const Eos& e, const double* l, const double* h) {
   double theta = 1.0;
   if (STAGE == 0) {
     theta = wmin(theta, theta_lower(l[0], h[0], PP_RHO));
@@ -315,17 +345,17 @@ __device__ __forceinline__ double theta_stage(const Eos& e, const double* l, con
 
 // ------------------------------------------------- reconstruction ----------
 template <typename T>
-__device__ __forceinline__ T mm2(T x, T y) {  // SC:295-297
+WAMP_FN T mm2(T x, T y) {  // SC:295-297
   return T(0.5) * (msign(x) + msign(y)) * wmin(fabs(x), fabs(y));
 }
 template <typename T>
-__device__ __forceinline__ T mm4(T a, T b, T c, T d) {  // SC:299-304
+WAMP_FN T mm4(T a, T b, T c, T d) {  // SC:299-304
   T s = msign(a) + msign(b);
   return T(0.125) * s * fabs((msign(a) + msign(c)) * (msign(a) + msign(d))) *
          wmin(wmin(fabs(a), fabs(b)), wmin(fabs(c), fabs(d)));
 }
 // cell_entropy SC:308-314
-__device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, double p, double a1) {
+WAMP_FN double entropy(const Eos& e, double ar1, double ar2, double p, double a1) {
   double rho = ar1 + ar2;
   if (rho < TINY) rho = TINY;
   double g, pi;
@@ -342,14 +372,14 @@ __device__ __forceinline__ double entropy(const Eos& e, double ar1, double ar2, 
 #endif
 }
 // cator2_val SC:318-325
-__device__ __forceinline__ double cator2(double em2, double em1, double e0, double ep1, double ep2) {
+WAMP_FN double cator2(double em2, double em1, double e0, double ep1, double ep2) {
   double aa = 0.25 * fabs(em2 - 4.0 * em1 + 3.0 * e0) + (13.0 / 12.0) * fabs(e0 - 2.0 * em1 + em2);
   double bb = 0.25 * fabs(3.0 * e0 - 4.0 * ep1 + ep2) + (13.0 / 12.0) * fabs(e0 - 2.0 * ep1 + ep2);
   return ddiv(2.0 * aa * bb + YUXIN, aa * aa + bb * bb + YUXIN);
 }
 // thinc_ul SC:327-337; t1 = tanh(beta/2) is a per-launch constant
 template <typename T>
-__device__ __forceinline__ T thinc_l(T vm1, T v0, T v1, T t1) {
+WAMP_FN T thinc_l(T vm1, T v0, T v1, T t1) {
   T result = v0;
   if ((v1 - v0) * (v0 - vm1) > T(0)) {
     T qa = T(0.5) * (v1 + vm1);
@@ -362,7 +392,7 @@ __device__ __forceinline__ T thinc_l(T vm1, T v0, T v1, T t1) {
 }
 // thinc_ur SC:339-349
 template <typename T>
-__device__ __forceinline__ T thinc_r(T v0, T v1, T v2, T t1) {
+WAMP_FN T thinc_r(T v0, T v1, T v2, T t1) {
   T result = v1;
   if ((v2 - v1) * (v1 - v0) > T(0)) {
     T qa = T(0.5) * (v2 + v0);
@@ -375,7 +405,7 @@ __device__ __forceinline__ T thinc_r(T v0, T v1, T v2, T t1) {
 }
 // mp5_ul SC:353-378 (the blend of SC:371-375 is overwritten by SC:376)
 template <typename T>
-__device__ __forceinline__ T mp5_l(T vm2, T vm1, T v0, T vp1, T vp2) {
+WAMP_FN T mp5_l(T vm2, T vm1, T v0, T vp1, T vp2) {
   T VOR = T(ONE_60) * (T(2) * vm2 - T(13) * vm1 + T(47) * v0 + T(27) * vp1 - T(3) * vp2);
   T VMP = v0 + mm2(vp1 - v0, T(MP5C) * (v0 - vm1));
   T result = VOR;
@@ -397,7 +427,7 @@ __device__ __forceinline__ T mp5_l(T vm2, T vm1, T v0, T vp1, T vp2) {
 }
 // mp5_ur SC:380-405
 template <typename T>
-__device__ __forceinline__ T mp5_r(T vm1, T v0, T vp1, T vp2, T vp3) {
+WAMP_FN T mp5_r(T vm1, T v0, T vp1, T vp2, T vp3) {
   T VOR = T(ONE_60) * (T(-3) * vm1 + T(27) * v0 + T(47) * vp1 - T(13) * vp2 + T(2) * vp3);
   T VMP = vp1 + mm2(v0 - vp1, T(MP5C) * (vp1 - vp2));
   T result = VOR;
@@ -419,14 +449,14 @@ __device__ __forceinline__ T mp5_r(T vm1, T v0, T vp1, T vp2, T vp3) {
 }
 // central6_avg SC:407-412
 template <typename T>
-__device__ __forceinline__ T central6(T vm2, T vm1, T v0, T vp1, T vp2, T vp3) {
+WAMP_FN T central6(T vm2, T vm1, T v0, T vp1, T vp2, T vp3) {
   T left = T(ONE_60) * (T(2) * vm2 - T(13) * vm1 + T(47) * v0 + T(27) * vp1 - T(3) * vp2);
   T right = T(ONE_60) * (T(-3) * vm1 + T(27) * v0 + T(47) * vp1 - T(13) * vp2 + T(2) * vp3);
   return T(0.5) * (left + right);
 }
 // 4th-order central interface value, SC:1077-1080
 template <typename T>
-__device__ __forceinline__ T central4(T vm1, T v0, T vp1, T vp2) {
+WAMP_FN T central4(T vm1, T v0, T vp1, T vp2) {
   return (-T(1.0 / 12.0) * vm1 + T(7.0 / 12.0) * v0 + T(7.0 / 12.0) * vp1 - T(1.0 / 12.0) * vp2);
 }
 // muscl_char_ul / muscl_char_ur SC:430-446:
@@ -435,20 +465,20 @@ __device__ __forceinline__ T central4(T vm1, T v0, T vp1, T vp2) {
 // zeros), so 0.5 (sign(d) + sign(e)) is formed once; every value is the one
 // mm2 gives.
 template <typename T>
-__device__ __forceinline__ T muscl_pair(T d, T e) {
+WAMP_FN T muscl_pair(T d, T e) {
   const T s = T(0.5) * (msign(d) + msign(e));
   const T a = s * wmin(fabs(d), fabs(T(2) * e));  // mm2(d, 2 e)
   const T b = s * wmin(fabs(e), fabs(T(2) * d));  // mm2(e, 2 d)
   return T(0.25) * ((T(1) - T(KAP)) * a + (T(1) + T(KAP)) * b);
 }
 template <typename T>
-__device__ __forceinline__ T muscl_l(T vm1, T v0, T vp1) {
+WAMP_FN T muscl_l(T vm1, T v0, T vp1) {
   T dm = v0 - vm1;
   T d0 = vp1 - v0;
   return v0 + muscl_pair(dm, d0);
 }
 template <typename T>
-__device__ __forceinline__ T muscl_r(T v0, T vp1, T vp2) {
+WAMP_FN T muscl_r(T v0, T vp1, T vp2) {
   T dp = vp2 - vp1;
   T d0 = vp1 - v0;
   return vp1 - muscl_pair(dp, d0);
@@ -462,7 +492,7 @@ struct Basis {
 };
 // DIR 0: x-face (normal u, SC:480-534); DIR 1: y-face (normal v, SC:537-589)
 template <int DIR, typename T>
-__device__ __forceinline__ Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg, T v_avg) {
+WAMP_FN Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg, T v_avg) {
   Basis<T> b;
   b.c = c;
   b.inv2c = dv(T(0.5), c);
@@ -484,7 +514,7 @@ __device__ __forceinline__ Basis<T> make_basis(T c, T rho, T ar1, T ar2, T u_avg
 }
 // L . u  (to_char_x SC:480-499 / to_char_y SC:537-556)
 template <int DIR, typename T>
-__device__ __forceinline__ void to_char(const Basis<T>& b, const T* u, T* w) {
+WAMP_FN void to_char(const Basis<T>& b, const T* u, T* w) {
   const T un_m = DIR == 0 ? u[2] : u[3];  // normal momentum
   const T ut_m = DIR == 0 ? u[3] : u[2];  // tangential momentum
   w[0] = b.k * u[0] + b.k * u[1] - b.inv2c * un_m + b.inv2c2 * u[4];
@@ -499,7 +529,7 @@ __device__ __forceinline__ void to_char(const Basis<T>& b, const T* u, T* w) {
 }
 // R . w  (to_phys_x SC:504-534 / to_phys_y SC:559-589)
 template <int DIR, typename T>
-__device__ __forceinline__ void to_phys(const Basis<T>& b, const T* w, T* s) {
+WAMP_FN void to_phys(const Basis<T>& b, const T* w, T* s) {
   const T u_avg = b.un_, v_avg = b.ut_, c = b.c;
   s[0] = b.Y1 * (w[0] + w[5]) + w[1];
   s[1] = b.Y2 * (w[0] + w[5]) + w[2];
@@ -516,14 +546,14 @@ __device__ __forceinline__ void to_phys(const Basis<T>& b, const T* w, T* s) {
 
 // ----------------------------------------------- Wong face limiter ---------
 // rho_ctilde2_sc SC:174-178
-__device__ __forceinline__ double rho_ct2_sc(const Eos& e, const double* s) {
+WAMP_FN double rho_ct2_sc(const Eos& e, const double* s) {
   double a1 = clamp01_pp(s[5]);
   double g, pi;
   mix_g_pi(e, a1, g, pi);
   return ddiv(s[4] + pi, g - 1.0);
 }
 // wong_stage_lower_s6 SC:189-201 (in place on s)
-__device__ __forceinline__ void wong_lower(const double* b, double* s, double bv, double hv,
+WAMP_FN void wong_lower(const double* b, double* s, double bv, double hv,
                                            double floor_) {
   double theta = 1.0;
   if (isnan_(bv) || isnan_(hv) || bv < floor_) {
@@ -543,7 +573,7 @@ __device__ __forceinline__ void wong_lower(const double* b, double* s, double bv
 // the later checks and recomputed only in the (rare) branches that change p
 // or alpha1, so every check sees the value a fresh rho_ctilde2_sc of the
 // current state gives.
-__device__ __forceinline__ void face_state_pp(const Eos& e, double* hi, const double* base) {
+WAMP_FN void face_state_pp(const Eos& e, double* hi, const double* base) {
   double b[6];
 #pragma unroll
   for (int m = 0; m < 6; ++m) b[m] = base[m];
@@ -593,7 +623,7 @@ __device__ __forceinline__ void face_state_pp(const Eos& e, double* hi, const do
 // one evaluation of cL, cR, SL, SR, SP feeds the flux and the contact
 // velocity u*.  Inputs are primitives (ar1, ar2, u, v, p, a1) per side.
 template <int DIR, typename T>
-__device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
+WAMP_FN void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
   const T ar1L = L[0], ar2L = L[1], uL = L[2], vL = L[3], pL = L[4], a1L = L[5];
   const T ar1R = R[0], ar2R = R[1], uR = R[2], vR = R[3], pR = R[4], a1R = R[5];
   const T nL = DIR == 0 ? uL : vL, nR = DIR == 0 ? uR : vR;
@@ -670,7 +700,7 @@ __device__ __forceinline__ void hllc(const Eos& e, const T* L, const T* R, T* f,
 // nondimensional water against p ~ 1), so an FP32 energy flux would put
 // ~eps32 * pb of noise into E and hence into the recovered pressure.
 template <int DIR>
-__device__ __forceinline__ void hllc_mixed(const Eos& e, const float* L, const float* R, double* f,
+WAMP_FN void hllc_mixed(const Eos& e, const float* L, const float* R, double* f,
                                            double& ustar) {
   const float ar1L = L[0], ar2L = L[1], uL = L[2], vL = L[3], pL = L[4], a1L = L[5];
   const float ar1R = R[0], ar2R = R[1], uR = R[2], vR = R[3], pR = R[4], a1R = R[5];

@@ -19,7 +19,7 @@ WAMP_DEV_BEGIN
 // into every y-ghost layer (SC:1664-1682), so corner cells hold the corner
 // interior cell.  Slip walls negate the wall-normal momentum.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ size_t map_cell(const Geo& G, int i, int j, double& sru, double& srv) {
+WAMP_FN size_t map_cell(const Geo& G, int i, int j, double& sru, double& srv) {
   sru = 1.0;
   srv = 1.0;
   if (i < G.g) {
@@ -66,11 +66,11 @@ struct FaceSmem {
 
 // Ducros x pressure sensor (ducros_kernel SC:875-916) at LDS cell r.
 template <class TL>
-__device__ __forceinline__ double lds_vel(const FaceSmem<TL>& S, int r, int comp) {
+WAMP_FN double lds_vel(const FaceSmem<TL>& S, int r, int comp) {
   return dv(S.q[comp][r], S.q[0][r] + S.q[1][r]);
 }
 template <class TL, int DIR>
-__device__ __forceinline__ double ducros_sigma(const FaceSmem<TL>& S, int r) {
+WAMP_FN double ducros_sigma(const FaceSmem<TL>& S, int r) {
   constexpr int FRJ = TL::RJ;
   double der_ux = 0.5 * (-lds_vel(S, r - FRJ, 2) + lds_vel(S, r + FRJ, 2));
   double der_vx = 0.5 * (-lds_vel(S, r - FRJ, 3) + lds_vel(S, r + FRJ, 3));
@@ -91,7 +91,7 @@ __device__ __forceinline__ double ducros_sigma(const FaceSmem<TL>& S, int r) {
 // along the normal are r0 + k*s.  ci / sg: indicator and Ducros sensor at
 // the cells -1..2.  Writes 6 fluxes + u* to out.
 template <class TL, int DIR, int BASE, int SHEAR, int FACEPP, typename T>
-__device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, int r0, const double* ci,
+WAMP_FN void face_high(const Eos& e, const FaceSmem<TL>& S, int r0, const double* ci,
                                           const double* sg, int vf_always, double t1d, double* out) {
   constexpr int s = DIR == 0 ? TL::RJ : 1;
   constexpr bool wide = BASE == 1 || SHEAR == 2;  // needs cells -2 and +3
@@ -219,7 +219,7 @@ __device__ __forceinline__ void face_high(const Eos& e, const FaceSmem<TL>& S, i
 
 // First-order flux (fx_low_kernel SC:1303-1347 / gy_low_kernel SC:1350-1394).
 template <class TL, int DIR, typename T>
-__device__ __forceinline__ void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) {
+WAMP_FN void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) {
   constexpr int s = DIR == 0 ? TL::RJ : 1;
   const int rL = r0, rR = r0 + s;
   double rhoL = S.q[0][rL] + S.q[1][rL];
@@ -246,56 +246,56 @@ __device__ __forceinline__ void face_low(const Eos& e, const FaceSmem<TL>& S, in
   }
 }
 
-// One tile of face positions (bi, bj) of one case: stage, indicators, faces.
+// One tile of face positions (bi, bj) of one case in three phases, each a
+// function of one work index so that the device runs a phase with one
+// thread per index (a barrier between phases) and the host with a loop:
+//   A  stage cell r of the rectangle: conserved values, pressure and
+//      entropy once per cell (r < TL::RN);
+//   B  entropy indicator (cator2_val SC:318-325) and Ducros sensor once per
+//      cell along each sweep direction (k < TL::CXN, k < TL::CYN);
+//   C  the x-face and the y-face at tile position t (t < TL::THREADS).
+template <class TL>
+WAMP_FN void tile_stage_cell(const Geo& G, const Eos& e, const State& U, int i0, int j0, int r,
+                             FaceSmem<TL>& S) {
+  constexpr int FRJ = TL::RJ;
+  const int i = i0 - 3 + r / FRJ;
+  const int j = j0 - 3 + r % FRJ;
+  double sru, srv;
+  const size_t c = map_cell(G, i, j, sru, srv);
+  double ar1 = U.f[0][c], ar2 = U.f[1][c];
+  double ru = U.f[2][c], rv = U.f[3][c];
+  double E = U.f[4][c], al = U.f[5][c];
+  if (sru < 0.0) ru = -1.0 * ru;
+  if (srv < 0.0) rv = -1.0 * rv;
+  double uu, vv;
+  double p = pressure(e, ar1, ar2, ru, rv, E, al, uu, vv);
+  S.q[0][r] = ar1;
+  S.q[1][r] = ar2;
+  S.q[2][r] = ru;
+  S.q[3][r] = rv;
+  S.q[4][r] = p;
+  S.q[5][r] = al;
+  S.ent[r] = entropy(e, ar1, ar2, p, al);
+}
+template <class TL, int SHEAR>
+WAMP_FN void tile_sensor_x(FaceSmem<TL>& S, int k) {
+  constexpr int FTJ = TL::TJ, FRJ = TL::RJ;
+  const int r = (k / FTJ + 2) * FRJ + (k % FTJ + 3);
+  S.cx[k] = cator2(S.ent[r - 2 * FRJ], S.ent[r - FRJ], S.ent[r], S.ent[r + FRJ], S.ent[r + 2 * FRJ]);
+  if (SHEAR != 0) S.sx[k] = ducros_sigma<TL, 0>(S, r);
+}
+template <class TL, int SHEAR>
+WAMP_FN void tile_sensor_y(FaceSmem<TL>& S, int k) {
+  constexpr int FTJ = TL::TJ, FRJ = TL::RJ;
+  const int r = (k / (FTJ + 3) + 3) * FRJ + (k % (FTJ + 3) + 2);
+  S.cy[k] = cator2(S.ent[r - 2], S.ent[r - 1], S.ent[r], S.ent[r + 1], S.ent[r + 2]);
+  if (SHEAR != 0) S.sy[k] = ducros_sigma<TL, 1>(S, r);
+}
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
-__device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const State& U, const Faces& F,
-                                           int vf_always, double t1, int bi, int bj,
-                                           FaceSmem<TL>& S) {
-  constexpr int FTI = TL::TI, FTJ = TL::TJ, FRJ = TL::RJ, FRN = TL::RN;
-  constexpr int FCXN = TL::CXN, FCYN = TL::CYN;
-  const int tid = threadIdx.x;
-  const int i0 = G.g - 1 + bi * FTI;
-  const int j0 = G.g - 1 + bj * FTJ;
-
-  // Phase A: stage the cells of the rectangle, pressure and entropy once.
-  for (int r = tid; r < FRN; r += FTHREADS) {
-    const int i = i0 - 3 + r / FRJ;
-    const int j = j0 - 3 + r % FRJ;
-    double sru, srv;
-    const size_t c = map_cell(G, i, j, sru, srv);
-    double ar1 = U.f[0][c], ar2 = U.f[1][c];
-    double ru = U.f[2][c], rv = U.f[3][c];
-    double E = U.f[4][c], al = U.f[5][c];
-    if (sru < 0.0) ru = -1.0 * ru;
-    if (srv < 0.0) rv = -1.0 * rv;
-    double uu, vv;
-    double p = pressure(e, ar1, ar2, ru, rv, E, al, uu, vv);
-    S.q[0][r] = ar1;
-    S.q[1][r] = ar2;
-    S.q[2][r] = ru;
-    S.q[3][r] = rv;
-    S.q[4][r] = p;
-    S.q[5][r] = al;
-    S.ent[r] = entropy(e, ar1, ar2, p, al);
-  }
-  __syncthreads();
-
-  // Phase B: entropy indicator (cator2_val SC:318-325) and Ducros sensor
-  // once per cell, along each sweep direction.
-  for (int k = tid; k < FCXN; k += FTHREADS) {
-    const int r = (k / FTJ + 2) * FRJ + (k % FTJ + 3);
-    S.cx[k] = cator2(S.ent[r - 2 * FRJ], S.ent[r - FRJ], S.ent[r], S.ent[r + FRJ], S.ent[r + 2 * FRJ]);
-    if (SHEAR != 0) S.sx[k] = ducros_sigma<TL, 0>(S, r);
-  }
-  for (int k = tid; k < FCYN; k += FTHREADS) {
-    const int r = (k / (FTJ + 3) + 3) * FRJ + (k % (FTJ + 3) + 2);
-    S.cy[k] = cator2(S.ent[r - 2], S.ent[r - 1], S.ent[r], S.ent[r + 1], S.ent[r + 2]);
-    if (SHEAR != 0) S.sy[k] = ducros_sigma<TL, 1>(S, r);
-  }
-  __syncthreads();
-
-  // Phase C: one x-face and one y-face per thread.
-  const int ti = tid / FTJ, tj = tid % FTJ;
+WAMP_FN void tile_faces_at(const Geo& G, const Eos& e, const Faces& F, int vf_always, double t1,
+                           int i0, int j0, int t, const FaceSmem<TL>& S) {
+  constexpr int FTJ = TL::TJ, FRJ = TL::RJ;
+  const int ti = t / FTJ, tj = t % FTJ;
   const int i = i0 + ti, j = j0 + tj;
   const int r0 = (ti + 3) * FRJ + (tj + 3);
   const size_t o = (size_t)i * (size_t)G.ld + (size_t)j;
@@ -327,6 +327,22 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
 #pragma unroll
     for (int m = 0; m < 7; ++m) fstore<T>(F, 21 + m, o, out[m]);
   }
+}
+
+// Tile (bi, bj) by the 256 threads of a block.
+template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
+__device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const State& U, const Faces& F,
+                                           int vf_always, double t1, int bi, int bj,
+                                           FaceSmem<TL>& S) {
+  const int tid = threadIdx.x;
+  const int i0 = G.g - 1 + bi * TL::TI;
+  const int j0 = G.g - 1 + bj * TL::TJ;
+  for (int r = tid; r < TL::RN; r += FTHREADS) tile_stage_cell<TL>(G, e, U, i0, j0, r, S);
+  __syncthreads();
+  for (int k = tid; k < TL::CXN; k += FTHREADS) tile_sensor_x<TL, SHEAR>(S, k);
+  for (int k = tid; k < TL::CYN; k += FTHREADS) tile_sensor_y<TL, SHEAR>(S, k);
+  __syncthreads();
+  tile_faces_at<TL, BASE, SHEAR, FACEPP, T>(G, e, F, vf_always, t1, i0, j0, tid, S);
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>

@@ -97,6 +97,16 @@ SIGNATURES = [
     ("wamp_timing_read", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("wamp_selftest_arith", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
     ("wamp_selftest_elementary", C.c_int, [_VP, _VP, _VP, C.c_int64, _VP]),
+    ("wamp_cpu_ctx_create", C.c_int, [C.POINTER(WampGrid), C.POINTER(WampEos),
+                                      C.POINTER(WampScheme), C.c_int32, C.POINTER(_VP)]),
+    ("wamp_cpu_ctx_destroy", C.c_int, [_VP]),
+    ("wamp_cpu_threads", C.c_int32, [_VP]),
+    ("wamp_cpu_prepare", C.c_int, [_VP, PtrArr6]),
+    ("wamp_cpu_apply_bc", C.c_int, [_VP, PtrArr6, _VP]),
+    ("wamp_cpu_cons_to_prim", C.c_int, [_VP, PtrArr6, PtrArr3]),
+    ("wamp_cpu_compute_fluxes", C.c_int, [_VP, PtrArr6, PtrArr28]),
+    ("wamp_cpu_get_dt", C.c_int, [_VP, PtrArr6, C.POINTER(C.c_double)]),
+    ("wamp_cpu_step", C.c_int, [_VP, PtrArr6, C.c_double, C.POINTER(C.c_double)]),
     ("wamp3_ctx_create", C.c_int, [C.POINTER(WampGrid3), C.POINTER(WampEos), C.POINTER(WampScheme),
                                    C.c_int32, C.POINTER(_VP)]),
     ("wamp3_ctx_destroy", C.c_int, [_VP]),

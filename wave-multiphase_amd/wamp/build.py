@@ -33,6 +33,7 @@ UNITS = [
     ("wamp3_faces_mixed.hip", "-ffp-contract=on"),
     ("wamp_capi.cpp", "-ffp-contract=off"),
     ("wamp_noalloc.hip", "-ffp-contract=off"),
+    ("wamp_cpu.hip", "-ffp-contract=off"),  # the CPU device: host code only
 ]
 
 

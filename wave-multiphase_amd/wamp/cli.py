@@ -1,18 +1,18 @@
 """Script entry point with the reference's CLI (SC:13-21) and time loop (SC:1997-2082).
 
-    python wave-multiphase_amd/wamp/cli.py [--cpu | --cuda] [--restart FILE.npz]
-        [--restart-time T] [--restart-step N]
+    python wave-multiphase_amd/wamp/cli.py [--cpu [--threads N] | --cuda]
+        [--restart FILE.npz] [--restart-time T] [--restart-step N]
         [--case shock_cylinder|mach10_cylinder|multi_droplet|triple_point]
         [--nx NX --ny NY] [--scheme sc|m10|mp5] [--t-end T] [--save-times T1,T2,...]
         [--max-steps N] [--out DIR] [--prefix NAME] [--png]
 
 The reference's flags keep their meaning: --cuda selects the GPU (the
-default), --restart/--restart-time/--restart-step restart from an npz file.
---cpu selected Warp's CPU backend in the reference; this build has no CPU
-device for the step, so --cpu exits with status 2 and a message.
-Everything the reference fixes in its source (grid, case, scheme, end time,
-save times) is a flag here.  Outputs are `{prefix}_{step:06d}.npz` as
-SC:1714 / M10:1636.
+default), --cpu the host (wamp.cpu.CpuSolver: the FP64 step's own code
+compiled for the host, on --threads worker threads), and
+--restart/--restart-time/--restart-step restart from an npz file.  --cpu
+computes in FP64, as the reference's CPU run does.  Everything the reference
+fixes in its source (grid, case, scheme, end time, save times) is a flag
+here.  Outputs are `{prefix}_{step:06d}.npz` as SC:1714 / M10:1636.
 """
 from __future__ import annotations
 
@@ -40,6 +40,9 @@ def parse(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--cpu", action="store_true")
     ap.add_argument("--cuda", action="store_true")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="worker threads of the CPU device (--cpu); 0: OMP_NUM_THREADS when "
+                         "set, else the CPUs this process may run on")
     ap.add_argument("--restart", type=str, default="")
     ap.add_argument("--restart-time", type=float, default=None)
     ap.add_argument("--restart-step", type=int, default=None)
@@ -62,12 +65,9 @@ def parse(argv=None):
 
 def main(argv=None) -> int:
     args = parse(argv)
-    if args.cpu:
-        print("--cpu: this build runs the solver step on the GPU only (HIP, gfx950); "
-              "there is no CPU device.  Run without --cpu on a ROCm GPU.", file=sys.stderr)
+    if args.cpu and args.cuda:
+        print("--cpu and --cuda select different devices; give one", file=sys.stderr)
         return 2
-    from wamp.solver import Solver
-
     make, (nx0, ny0), scheme0, t_end0 = CASES[args.case]
     scheme_name = args.scheme or scheme0
     nx, ny = args.nx or nx0, args.ny or ny0
@@ -77,9 +77,18 @@ def main(argv=None) -> int:
     prefix = args.prefix or ("mach10" if scheme_name == "m10" else "mp5")
     os.makedirs(args.out, exist_ok=True)
     g = cfg.grid
-    print(f"Device: cuda  NX={g.nx}  NY={g.ny}  G={g.g}  case={args.case}  scheme={scheme_name}")
+    if args.cpu:
+        from wamp.cpu import CpuSolver
 
-    s = Solver(cfg)
+        s = CpuSolver(cfg, threads=args.threads)
+        device = f"cpu ({s.threads} threads)"
+    else:
+        from wamp.solver import Solver
+
+        s = Solver(cfg)
+        device = "cuda"
+    print(f"Device: {device}  NX={g.nx}  NY={g.ny}  G={g.g}  case={args.case}  scheme={scheme_name}")
+
     restart = args.restart != ""
     if restart:
         U, t_file = io.load_restart(args.restart, cfg)

@@ -144,32 +144,45 @@ class Solver:
             self.step_device()
         return dts.cpu().numpy()
 
+    def step_dt(self, dt: float) -> float:
+        """One step with time step dt; returns the next dt (cfl * min of the new state)."""
+        self.set_dt(dt)
+        self.step_device()
+        return float(self.dt_buf[0].item())
+
     def advance_to(self, t_end: float, save_times=(), on_save=None, log_every: int = 200,
                    log=print, max_steps: int | None = None):
         """Time loop of main() (SC:2010-2073): dt clipped to save times and t_end."""
-        import time as _time
+        return advance(self, t_end, save_times, on_save, log_every, log, max_steps)
 
-        save_flags = [self.time >= ts - 1.0e-14 for ts in save_times]
-        dt = self.get_dt()
-        t0 = _time.time()
-        while self.time < t_end - 1.0e-14:
-            if max_steps is not None and self.nstep >= max_steps:
-                break
-            for k, ts in enumerate(save_times):
-                if not save_flags[k] and self.time + dt > ts:
-                    dt = ts - self.time
-            if self.time + dt > t_end:
-                dt = t_end - self.time
-            self.set_dt(dt)
-            self.step_device()
-            self.time += dt
-            dt = float(self.dt_buf[0].item())
-            if log_every and self.nstep % log_every == 0 and log:
-                log(f"  step {self.nstep:5d}  t={self.time:.4e}  dt={dt:.3e}  "
-                    f"wall={_time.time() - t0:.1f}s")
-            for k, ts in enumerate(save_times):
-                if not save_flags[k] and abs(self.time - ts) < 0.5 * dt:
-                    if on_save:
-                        on_save(self)
-                    save_flags[k] = True
-        return self.time
+
+def advance(s, t_end: float, save_times=(), on_save=None, log_every: int = 200, log=print,
+            max_steps: int | None = None):
+    """Time loop of main() (SC:2010-2073) over a solver `s` with `time`, `nstep`,
+    get_dt() and step_dt(dt) -> next dt: dt clipped to the save times and t_end,
+    on_save(s) at each save time."""
+    import time as _time
+
+    save_flags = [s.time >= ts - 1.0e-14 for ts in save_times]
+    dt = s.get_dt()
+    t0 = _time.time()
+    while s.time < t_end - 1.0e-14:
+        if max_steps is not None and s.nstep >= max_steps:
+            break
+        for k, ts in enumerate(save_times):
+            if not save_flags[k] and s.time + dt > ts:
+                dt = ts - s.time
+        if s.time + dt > t_end:
+            dt = t_end - s.time
+        step_dt = dt
+        dt = s.step_dt(step_dt)
+        s.time += step_dt
+        if log_every and s.nstep % log_every == 0 and log:
+            log(f"  step {s.nstep:5d}  t={s.time:.4e}  dt={dt:.3e}  "
+                f"wall={_time.time() - t0:.1f}s")
+        for k, ts in enumerate(save_times):
+            if not save_flags[k] and abs(s.time - ts) < 0.5 * dt:
+                if on_save:
+                    on_save(s)
+                save_flags[k] = True
+    return s.time
