@@ -15,7 +15,8 @@ value = NX*NY*steps / t / 1e9.
 
 Also reported, for the dominant kernel (the face-flux kernel), its average
 launch duration from hipEvents recorded around it inside the timed region,
-and as cpu_baseline the CPU oracle on the host cores over a bounded sample.
+and as cpu_baseline the CPU oracle on the host cores over a bounded sample;
+cpu_device is the product's own CPU device (--cpu) on the same case.
 """
 from __future__ import annotations
 
@@ -69,6 +70,31 @@ def cpu_baseline(cfg, U0, steps: int):
         "kind": "port",
         "sample": f"{steps} SSP-RK3 step(s) of the same {g.nx}x{g.ny} case, C restatement "
                   f"of the reference kernels (OpenMP, {th} threads), {el:.1f} s",
+    }
+
+
+def cpu_device_leg(cfg, U0, steps: int):
+    """The product's CPU device (--cpu: the FP64 step's own code on the host) on the
+    same workload: `steps` steps, the threads the process may use."""
+    from wamp.cpu import CpuSolver
+
+    th = cpu_threads()
+    s = CpuSolver(cfg, threads=th)
+    s.set_state(U0)
+    s.prepare()
+    dt = s.get_dt()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = s.step_dt(dt)
+    el = time.perf_counter() - t0
+    s.close()
+    g = cfg.grid
+    return {
+        "value": g.nx * g.ny * steps / el / 1e9,
+        "unit": "Gcell-updates/s",
+        "threads": th,
+        "sample": f"{steps} SSP-RK3 step(s) of the same {g.nx}x{g.ny} case on the CPU device "
+                  f"(wamp_cpu_step, {th} threads), {el:.1f} s",
     }
 
 
@@ -485,6 +511,8 @@ def main(argv=None):
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=5,
                     help="steps of the CPU oracle sample (0 = skip the cpu_baseline leg)")
+    ap.add_argument("--cpu-device-steps", type=int, default=2,
+                    help="steps of the product CPU device (--cpu) on the same case (0 = skip)")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the FP32-mixed leg on the same case (0 = skip)")
     ap.add_argument("--batch-cases", type=int, default=64,
@@ -636,6 +664,9 @@ def main(argv=None):
         out["cpu_baseline"] = cpu_baseline(cfg, U0, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
+    if args.cpu_device_steps > 0:
+        out["cpu_device"] = cpu_device_leg(cfg, U0, args.cpu_device_steps)
+        _progress(f"CPU device {out['cpu_device']['value']:.5f} Gcell-updates/s")
     print(json.dumps(out), flush=True)
     return 0
 

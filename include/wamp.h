@@ -146,7 +146,9 @@ int wamp_step_noalloc(const wamp_grid *grid, const wamp_eos *eos, const wamp_sch
    face tile phases, the limiter stages and the update are host-and-device
    functions), with the IEEE divide and square root that the kernels' division
    and square-root sequences reproduce, and the same pow and tanh -- the
-   result is the FP64 GPU step's, bit for bit.  Pointers are HOST pointers,
+   result is the FP64 GPU step's, bit for bit, except in the last bits of
+   subnormal values (below 2^-1022), where the kernels' shortened division
+   is not the IEEE quotient.  Pointers are HOST pointers,
    with the layout of the device entry points; every call returns when done.
    FP64 only (precision is not an argument).  One context is not re-entrant:
    calls on one context from several threads race.  No HIP call is made. */

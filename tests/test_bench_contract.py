@@ -39,3 +39,14 @@ def test_limiter_rates_from_the_committed_record():
     assert abs(r["traffic_gbs"] - r["traffic"] / 0.861e-3 / 1e9) < 0.1
     assert abs(r["traffic_frac_of_copy"] - r["traffic_gbs"] / 6000.0) < 1e-4
     assert bench.limiter_rates("mach10_cylinder_512x256", 0.1, 6000.0)["traffic"] is None
+
+
+def test_cpu_device_leg_runs_on_the_host():
+    sys.path.insert(0, ROOT)
+    import bench
+    from wamp import cases
+
+    cfg, U0 = cases.mach10_cylinder(64, 32)
+    r = bench.cpu_device_leg(cfg, U0, 2)
+    assert r["value"] > 0 and r["unit"] == "Gcell-updates/s" and r["threads"] >= 1
+    assert "64x32" in r["sample"]

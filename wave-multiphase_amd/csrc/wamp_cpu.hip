@@ -6,7 +6,9 @@
 // wamp_limit.h's per-cell limiter stages and update are host-and-device
 // functions.  Host builds of their FP64 arithmetic use the IEEE divide and
 // square root that the device sequences reproduce bit for bit, and the same
-// pow and tanh (wamp_math.h), so a host step gives the device step's bits.
+// pow and tanh (wamp_math.h), so a host step gives the device step's bits
+// (but for the last bits of subnormal values, below 2^-1022, where the
+// kernels' shortened division is not the IEEE quotient).
 // Only the drivers differ: a face tile is one loop per phase instead of 256
 // threads and barriers, the limiter runs each theta stage over whole planes,
 // and the work of a sweep is split over std::threads by rows or tiles.

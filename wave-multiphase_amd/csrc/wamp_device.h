@@ -26,8 +26,9 @@
 // The scalar helpers are host-and-device functions: the FP64 step also runs
 // on the host (wamp_cpu.hip, the CPU device) from this same source.  Device
 // builds use the hardware reciprocal / square-root sequences below, host
-// builds the IEEE operations those sequences reproduce bit for bit, so a
-// host step gives the device step's bits.
+// builds the IEEE operations those sequences reproduce bit for bit in their
+// operand range (below it -- subnormal quotients -- the last bits may
+// differ), so a host step gives the device step's bits.
 #define WAMP_FN __host__ __device__ __forceinline__
 
 // The FP64 kernels (wamp_kernels.hip) and the FP32-mixed face kernels

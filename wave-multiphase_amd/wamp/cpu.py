@@ -3,8 +3,9 @@
 `CpuSolver` has the interface of `wamp.solver.Solver` with the state in host
 memory (numpy) and libwamp.so's wamp_cpu_* entry points for every part of
 the step: the FP64 step's own per-cell code compiled for the host, which
-gives the FP64 GPU step's bits.  It is chosen explicitly (`--cpu` on the
-command line); nothing falls back to it.
+gives the FP64 GPU step's bits (subnormal values aside: below 2^-1022 the
+kernels' shortened division may round differently).  It is chosen
+explicitly (`--cpu` on the command line); nothing falls back to it.
 """
 from __future__ import annotations
 
