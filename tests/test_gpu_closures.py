@@ -288,6 +288,82 @@ def test_limit_update_matches_oracle(gpu_ok, lu_kind, forced, monkeypatch):
             assert got[m, G + i, G + j] == ref[m, G + i, G + j]
 
 
+def test_limiter_kernels_agree_under_heavy_limiting(gpu_ok, monkeypatch):
+    """wamp_limit_update on face fluxes whose high-order values are scaled by
+    seeded random factors in [-29, 31], so that the Wong stages limit about
+    15 % of the cells (SC:1397-1548), against the oracle's limit_fluxes +
+    rk3_update + enforce_admissible + update_prims, with the LDS-tile kernel
+    and the march at every segment length, all bitwise equal to each other.
+    The first output row i0 of a march segment depends on theta0 three rows
+    above it (update <- theta2(i0-1) <- theta1(i0-2) <- theta0(i0-3)), whose
+    lower face the march has to load before its first front."""
+    import torch
+
+    from oracle import oracle as O
+    from wamp import _lib
+    from wamp.solver import Solver
+
+    cfg, U0 = _case("m10", nx=160, ny=120)
+    g = cfg.grid
+    orc = O.Oracle(cfg)
+    orc.set_state(U0)
+    orc.prepare()
+    orc.run(12)
+    Un = orc.state()
+    dt = orc.get_dt()
+    orc.step(0.5 * dt)
+    U = orc.state()
+    orc.set_state(Un)
+    orc.copy_old()
+    orc.set_state(U)
+    orc.apply_bc()
+    orc.update_prims()
+    orc.apply_bc()
+    orc.compute_fluxes()
+    rng = np.random.default_rng(7)
+    for base in (O.FX0, O.GY0):
+        for m in range(7):
+            a = orc.array(base + m)
+            a *= 1.0 + 30.0 * rng.uniform(-1.0, 1.0, size=a.shape)
+    F = orc.fluxes()
+    orc.limit_fluxes(dt)
+    limited = float((orc.array(O.THETA)[g.interior()] < 1.0).mean())
+    orc.rk3_update(0.75, 0.25, dt)
+    orc.enforce_admissible()
+    orc.update_prims()
+    ref = orc.state()
+    orc.close()
+    assert limited > 0.05
+
+    si, sj = g.interior()
+    outs = {}
+    variants = [("0", "0")] + [("1", s) for s in ("4", "8", "16", "20", "26", "28", "32", "64")]
+    for kind, seg in variants:
+        monkeypatch.setenv("WAMP_LU_KIND", kind)
+        monkeypatch.setenv("WAMP_LU_SEG", seg)
+        s = Solver(cfg)
+        dev = s.device
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        Un_d, U_d, F_d = t(Un), t(U), t(F)
+        Uo = torch.zeros_like(U_d)
+        dt_d = torch.tensor([dt], dtype=torch.float64, device=dev)
+        _lib.check(s.L.wamp_limit_update(s.ctx, _p6(Un_d), _p6(U_d),
+                                         _lib.ptrs28([F_d[m].data_ptr() for m in range(28)]),
+                                         _p6(Uo), 0.75, 0.25, dt_d.data_ptr(), None, _stream()),
+                   "wamp_limit_update")
+        outs[(kind, seg)] = Uo.cpu().numpy()[:, si, sj]
+        s.close()
+    b = ref[:, si, sj]
+    a = outs[("0", "0")]
+    assert np.isfinite(a).all() and np.isfinite(b).all()
+    errs = [rel_l1(a[m], b[m]) for m in range(6)]
+    print(f"limited fraction {limited:.3f}; LDS-tile kernel vs oracle: rel L1",
+          ["%.1e" % e for e in errs])
+    assert max(errs) <= 1e-13
+    for k, st in outs.items():
+        assert np.array_equal(st, a), k
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("lu_kind", ["0", "1"])
 def test_repaired_state_steps_like_oracle(gpu_ok, lu_kind, precision, monkeypatch):

@@ -239,6 +239,19 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   double t2[3] = {1.0, 1.0, 1.0};            // theta2 rows f-4 .. f-2
   double local_dt = 1.0e30;
 
+  // The first theta0 row, i0-3, is part of the first output row's dependency
+  // cone (update(i0) <- theta2(i0-1) <- theta1(i0-2) <- theta0(i0-3)), so its
+  // lower x-face, row i0-4, is loaded before the march starts.
+  {
+    const int r = i0 - 4;
+    const bool ok = xrow_in(r) && col_in;
+    const size_t c = at(r);
+    x01[2][0] = face(ok, 0, c, r, 0);
+    x01[2][1] = face(ok, 1, c, r, 0);
+    x01[2][2] = face(ok, 14, c, r, 0);
+    x01[2][3] = face(ok, 15, c, r, 0);
+  }
+
   for (int f = i0 - 3; f < i_end + 3; ++f) {
     const int s3f = f % 3, s3m1 = (f + 2) % 3, s3m2 = (f + 1) % 3;  // slots of rows f, f-1, f-2
     const int s2m1 = (f + 1) & 1, s2m2 = f & 1;                      // rows f-1, f-2 (f-3 = f-1)
