@@ -337,7 +337,8 @@ def test_limiter_kernels_agree_under_heavy_limiting(gpu_ok, monkeypatch):
 
     si, sj = g.interior()
     outs = {}
-    variants = [("0", "0")] + [("1", s) for s in ("4", "8", "16", "20", "26", "28", "32", "64")]
+    variants = ([("0", "0")] + [("1", s) for s in ("4", "8", "16", "20", "26", "28", "32", "64")]
+                + [("2", s) for s in ("4", "8", "16", "26", "32", "48", "64")])
     for kind, seg in variants:
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)
@@ -365,7 +366,7 @@ def test_limiter_kernels_agree_under_heavy_limiting(gpu_ok, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("lu_kind", ["0", "1"])
+@pytest.mark.parametrize("lu_kind", ["0", "1", "2"])
 def test_repaired_state_steps_like_oracle(gpu_ok, lu_kind, precision, monkeypatch):
     """The bad cells repaired by the initial sequence (SC:1998-2001), then one
     step with either limiter/update kernel.  FP64: within 1e-12 of the oracle.

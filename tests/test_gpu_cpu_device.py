@@ -62,7 +62,7 @@ def test_cpu_device_fluxes_equal_gpu(gpu_ok, name):
     cpu.close()
 
 
-@pytest.mark.parametrize("lu_kind", ["0", "1"])
+@pytest.mark.parametrize("lu_kind", ["0", "1", "2"])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_cpu_device_steps_equal_gpu(gpu_ok, name, lu_kind, monkeypatch):
     from wamp.cpu import CpuSolver

@@ -135,7 +135,10 @@ def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, precision, monkeypatch):
     dt = orc.get_dt()
     outs = {}
     segs = ("4", "8", "16", "20", "26", "28", "32", "64")
-    for kind, seg in (("0", "32"),) + tuple(("1", s) for s in segs):
+    # the march that forms the first-order fluxes itself reads FP64 face storage only
+    segs_lo = ("4", "8", "16", "26", "32", "48", "64") if precision == 0 else ()
+    for kind, seg in ((("0", "32"),) + tuple(("1", s) for s in segs)
+                      + tuple(("2", s) for s in segs_lo)):
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)
         s = Solver(cfg)
@@ -165,7 +168,7 @@ def test_march_limiter_on_degenerate_grids(gpu_ok, nx, ny, precision, monkeypatc
     cfg = dataclasses.replace(cfg, precision=precision)
     U = orc.state()
     outs = {}
-    for kind, seg in (("0", "0"), ("1", "4"), ("1", "26")):
+    for kind, seg in (("0", "0"), ("1", "4"), ("1", "26"), ("2", "4"), ("2", "26")):
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)
         s = Solver(cfg)
@@ -212,7 +215,7 @@ def test_boundary_variants_match_oracle(gpu_ok, scheme, bc):
     assert max(errs) <= 1e-9, (bc, errs)
 
 
-@pytest.mark.parametrize("lu_kind", ["0", "1"])
+@pytest.mark.parametrize("lu_kind", ["0", "1", "2"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_padded_row_stride_gives_the_same_result(gpu_ok, precision, lu_kind, monkeypatch):
     """A state whose rows are padded (ld = ny + 2g + 13, not a multiple of

@@ -7,7 +7,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("lu_kind", ["0", "1"])
+@pytest.mark.parametrize("lu_kind", ["0", "1", "2"])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("scheme", ["sc", "m10", "mp5"])
 def test_split_calls_equal_one_step(gpu_ok, scheme, precision, lu_kind, monkeypatch):

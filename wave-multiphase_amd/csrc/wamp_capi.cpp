@@ -28,7 +28,7 @@ static Eos make_eos(const wamp_eos& e) { return make_eos_values(e.gamma1, e.pinf
 extern "C" {
 hipError_t wamp_launch_faces(const Geo*, const Eos*, const State*, const Faces*, int base,
                              int shear, int face_pp, int vf_always, double t1, int precision,
-                             hipStream_t);
+                             int low, hipStream_t);
 hipError_t wamp_launch_limit_update(const Geo*, const Eos*, const State* Un, const State* U,
                                     const Faces*, const State* Uo, double alpha, double beta,
                                     const double* dt_dev, double* dt_acc, int repair,
@@ -37,6 +37,10 @@ hipError_t wamp_launch_lu_march(const Geo*, const Eos*, const State* Un, const S
                                const Faces*, const State* Uo, double alpha, double beta,
                                const double* dt_dev, double* dt_acc, int repair, int seg,
                                hipStream_t);
+hipError_t wamp_launch_lu_lo(const Geo*, const Eos*, const State* Un, const State* U,
+                             const Faces*, const State* Uo, double alpha, double beta,
+                             const double* dt_dev, double* dt_acc, int repair, int seg,
+                             hipStream_t);
 hipError_t wamp_launch_dt(const Geo*, const Eos*, const State*, double* dt_acc, double* dt_out,
                           double cfl, hipStream_t);
 hipError_t wamp_launch_dt_finalize(double* dt_acc, double* dt_out, double cfl, hipStream_t);
@@ -343,30 +347,71 @@ static int lu_rows(const StepPlan* c) {
   return best;
 }
 
+// Rows per wave of the march that forms the first-order fluxes itself
+// (k_lu_lo): its blocks (4 waves of one strip) run one per CU and a wave of
+// `seg` rows runs seg + 6 fronts (seg + 7 when that is odd: the fronts run in
+// pairs), so the cost is rounds x (seg + 6) with one block slot per CU and 56
+// output columns per wave.  At 4096x2048 it picks 32 rows, the measured best
+// (ms per launch: 16 rows 1.118, 26 0.992, 32 0.966, 48 1.089, 64 1.027).
+constexpr int LO_COLS = 56;
+static int lu_rows_lo(const StepPlan* c) {
+  if (c->lu_seg) return c->lu_seg;
+  static const int cand[] = {8, 16, 26, 32, 48, 64};
+  const long long strips = (c->geo.ny + LO_COLS - 1) / LO_COLS;
+  const long long slots = std::max(1, c->cu_count);
+  int best = 16;
+  long long best_cost = -1;
+  for (int seg : cand) {
+    const long long segs = (c->geo.nx + seg - 1) / seg;
+    const long long blocks = strips * ((segs + 3) / 4);
+    const long long cost = ((blocks + slots - 1) / slots) * (seg + 6);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = seg;
+    }
+  }
+  return best;
+}
+
 // Limiter kernel: the LDS-tile kernel (every tile in parallel, four short
 // phases) has the lower latency on small grids, whose march would leave most
 // SIMDs idle; the marching kernel moves fewer bytes and wins from about
 // 2^18 cells (measured: 512x256 0.12 vs 0.19 ms per step, 1024x512 0.37 vs
-// 0.33, 2048x1024 1.45 vs 1.14).  Both give bitwise the same result.
-static int lu_kind(const StepPlan* c) {
-  if (c->lu_kind >= 0) return c->lu_kind;
-  return (long long)c->geo.nx * c->geo.ny <= (1LL << 18) ? 0 : 1;
+// 0.33, 2048x1024 1.45 vs 1.14).  Kind 2 is the march that forms the
+// first-order fluxes itself (FP64 face storage only), so the face kernel
+// skips them; it is the FP64 step's march (4096x2048 M10: faces 1.07 instead
+// of 1.27 ms, limiter 0.97 instead of 0.85 ms per launch, step 6.12 instead
+// of 6.36 ms).  Every kind gives bitwise the same result.  `planes`: the
+// caller hands over all 28 face planes (the split entry point), which kind 2
+// reads only when WAMP_LU_KIND asks for it.
+static int lu_kind(const StepPlan* c, bool planes = false) {
+  int k = c->lu_kind >= 0 ? c->lu_kind : (long long)c->geo.nx * c->geo.ny <= (1LL << 18) ? 0 : 2;
+  if (k == 2 && (c->precision != 0 || (planes && c->lu_kind != 2))) k = 1;
+  return k;
 }
 
-static int launch_lu(const StepPlan* c, const State& un, const State& u, const Faces& f,
+static int launch_lu(const StepPlan* c, int kind, const State& un, const State& u, const Faces& f,
                      const State& uo, double alpha, double beta, const double* dt_dev,
                      double* dt_acc, hipStream_t st) {
-  hipError_t e = lu_kind(c) == 1
-                     ? wamp_launch_lu_march(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev,
-                                            dt_acc, c->scheme.repair, lu_rows(c), st)
-                     : wamp_launch_limit_update(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta,
-                                                dt_dev, dt_acc, c->scheme.repair, st);
+  hipError_t e;
+  if (kind == 2)
+    e = wamp_launch_lu_lo(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev, dt_acc,
+                          c->scheme.repair, lu_rows_lo(c), st);
+  else if (kind == 1)
+    e = wamp_launch_lu_march(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev, dt_acc,
+                             c->scheme.repair, lu_rows(c), st);
+  else
+    e = wamp_launch_limit_update(&c->geo, &c->eos, &un, &u, &f, &uo, alpha, beta, dt_dev, dt_acc,
+                                 c->scheme.repair, st);
   return hip_check(e, "limit_update");
 }
 
-static int launch_faces(const StepPlan* c, const State& s, const Faces& f, hipStream_t st) {
+// low = 0: the face kernel skips the first-order fluxes (limiter kind 2).
+static int launch_faces(const StepPlan* c, const State& s, const Faces& f, int low,
+                        hipStream_t st) {
   return hip_check(wamp_launch_faces(&c->geo, &c->eos, &s, &f, c->scheme.base, c->scheme.shear,
-                                     c->scheme.face_pp, c->scheme.vf_thinc, c->t1, c->precision, st),
+                                     c->scheme.face_pp, c->scheme.vf_thinc, c->t1, c->precision,
+                                     low, st),
                    "faces");
 }
 
@@ -376,7 +421,7 @@ int wamp_compute_fluxes(wamp_ctx* c, double* const U[6], double* const F[28], vo
   if (int rc = state_of(U, s, "U")) return rc;
   Faces f;
   if (int rc = faces_of(c, F, f)) return rc;
-  return launch_faces(c, s, f, (hipStream_t)stream);
+  return launch_faces(c, s, f, 1, (hipStream_t)stream);
 }
 
 int wamp_limit_update(wamp_ctx* c, double* const Un[6], double* const U[6], double* const F[28],
@@ -392,7 +437,8 @@ int wamp_limit_update(wamp_ctx* c, double* const Un[6], double* const U[6], doub
   if (!dt_dev) return fail(-2, "dt_dev must be non-NULL");
   Faces f;
   if (int rc = faces_of(c, F, f)) return rc;
-  return launch_lu(c, sn, s, f, so, alpha, beta, dt_dev, dt_acc, (hipStream_t)stream);
+  return launch_lu(c, lu_kind(c, true), sn, s, f, so, alpha, beta, dt_dev, dt_acc,
+                   (hipStream_t)stream);
 }
 
 int wamp_get_dt(wamp_ctx* c, double* const U[6], double* dt_dev, void* stream) {
@@ -429,12 +475,13 @@ int wamp::step_enqueue(const StepPlan& p, const State& a, const StepScratch& scr
   // stage 2: U2 = 3/4 Un + 1/4 (U1 + dt R(U1))                (SC:2040-2046)
   // stage 3: Un+1 = 1/3 Un + 2/3 (U2 + dt R(U2)), written over Un, with the
   //          CFL minimum of the new state folded in           (SC:2049-2059)
+  const int kind = lu_kind(&p);
   if (int rc = mark(0)) return rc;
   for (int s = 0; s < 3; ++s) {
-    if (int rc = launch_faces(&p, *cur[s], scr.faces, st)) return rc;
+    if (int rc = launch_faces(&p, *cur[s], scr.faces, kind != 2, st)) return rc;
     if (int rc = mark(2 * s + 1)) return rc;
-    if (int rc = launch_lu(&p, a, *cur[s], scr.faces, *dst[s], coef[s][0], coef[s][1], dt_dev,
-                           s == 2 ? scr.dt_acc : nullptr, st))
+    if (int rc = launch_lu(&p, kind, a, *cur[s], scr.faces, *dst[s], coef[s][0], coef[s][1],
+                           dt_dev, s == 2 ? scr.dt_acc : nullptr, st))
       return rc;
     if (int rc = mark(2 * s + 2)) return rc;
   }

@@ -104,7 +104,7 @@ void faces_t(CpuCase& c, const State& U, const Faces& F) {
       for (int k = 0; k < CTile::CXN; ++k) tile_sensor_x<CTile, SHEAR>(S, k);
       for (int k = 0; k < CTile::CYN; ++k) tile_sensor_y<CTile, SHEAR>(S, k);
       for (int p = 0; p < CTile::THREADS; ++p)
-        tile_faces_at<CTile, BASE, SHEAR, FACEPP, double>(G, c.eos, F, vf, c.t1, i0, j0, p, S);
+        tile_faces_at<CTile, BASE, SHEAR, FACEPP, double>(G, c.eos, F, vf, c.t1, 1, i0, j0, p, S);
     }
   });
 }

@@ -296,6 +296,25 @@ WAMP_FN double pressure(const Eos& e, double ar1, double ar2, double ru,
   return ddiv(E - pb - 0.5 * rho * (uu * uu + vv * vv), gb);
 }
 
+// Primitive state of one cell as the first-order flux sees it (fx_low_kernel
+// SC:1303-1347): partial densities, velocities u = ru / rho, v = rv / rho,
+// the pressure floored where rho c~^2 < PP_CTILDE (SC:1335-1338) and the
+// clamped alpha.  The face kernels and the limiter march that forms the
+// first-order fluxes itself both go through this function.
+WAMP_FN void low_prims(const Eos& e, double ar1, double ar2, double uu, double vv, double p,
+                       double al, double* P) {
+  const double a = clamp01_pp(al);
+  double g, pi;
+  mix_g_pi(e, a, g, pi);
+  if (isnan_(p) || ddiv(p + pi, g - 1.0) < PP_CTILDE) p = PP_CTILDE * (g - 1.0) - pi;
+  P[0] = ar1;
+  P[1] = ar2;
+  P[2] = uu;
+  P[3] = vv;
+  P[4] = p;
+  P[5] = a;
+}
+
 // ------------------------------------------------------- positivity --------
 // pp_theta_lower SC:123-138
 WAMP_FN double theta_lower(double base, double high, double floor_) {
@@ -622,18 +641,26 @@ WAMP_FN void face_state_pp(const Eos& e, double* hi, const double* base) {
 // ---------------------------------------------------------------- HLLC -----
 // hllc_x + hllc_x_src (SC:594-687) / hllc_y + hllc_y_src (SC:690-771) fused:
 // one evaluation of cL, cR, SL, SR, SP feeds the flux and the contact
-// velocity u*.  Inputs are primitives (ar1, ar2, u, v, p, a1) per side.
+// velocity u*.  Inputs are primitives (ar1, ar2, u, v, p, a1) per side and
+// each side's sound speed c and total energy E (hllc below forms them; the
+// limiter march that forms first-order fluxes carries them per cell).
+//
+// The reference's four cases (SL > 0; SL <= 0 < SP; SP <= 0 <= SR; else)
+// take the left state in the first two and the right state in the last two,
+// and add the star-state correction S (U*_K - U_K) in the middle two.  Here
+// the side is selected first and one flux expression is evaluated: each
+// component is the reference's expression of its case on the selected
+// operands (the correction is selected in, not added as zero, so every
+// result equals the case's expression bit for bit), and lanes that take
+// different cases no longer run four bodies.
 template <int DIR, typename T>
-WAMP_FN void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
+WAMP_FN void hllc_ce(const Eos& e, const T* L, const T* R, T cL, T cR, T EL, T ER, T* f,
+                     T& ustar) {
   const T ar1L = L[0], ar2L = L[1], uL = L[2], vL = L[3], pL = L[4], a1L = L[5];
   const T ar1R = R[0], ar2R = R[1], uR = R[2], vR = R[3], pR = R[4], a1R = R[5];
   const T nL = DIR == 0 ? uL : vL, nR = DIR == 0 ? uR : vR;
   T rhoL = ar1L + ar2L;
   T rhoR = ar1R + ar2R;
-  T cL = sound_speed(e, rhoL, pL, a1L);
-  T cR = sound_speed(e, rhoR, pR, a1R);
-  T EL = prim_energy(e, ar1L, ar2L, uL, vL, pL, a1L);
-  T ER = prim_energy(e, ar1R, ar2R, uR, vR, pR, a1R);
   T n_avg = T(0.5) * (nL + nR);
   T c_avg = T(0.5) * (cL + cR);
   T SL = wmin(nL - cL, n_avg - c_avg);
@@ -644,55 +671,53 @@ WAMP_FN void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
   T ratL = (SL - nL) / (SL - SP);
   T ratR = (SR - nR) / (SR - SP);
 
-  // tangential momentum flux rho*u*v: x-face forms (rho*u)*v, y-face (rho*u)*v
-  // with u tangential -- both are rho*uL*vL in the reference's order.
-  if (SL > T(0)) {
-    f[0] = ar1L * nL;
-    f[1] = ar2L * nL;
-    if (DIR == 0) { f[2] = rhoL * uL * uL + pL; f[3] = rhoL * uL * vL; }
-    else          { f[2] = rhoL * uL * vL;      f[3] = rhoL * vL * vL + pL; }
-    f[4] = (EL + pL) * nL;
-    f[5] = a1L * nL;
-  } else if (SL <= T(0) && T(0) < SP) {
-    T ratio = ratL;
-    T EL_star = EL + (SP - nL) * (SP * rhoL + dv(pL, SL - nL));  // SL - nL <= -cL < 0
-    f[0] = ar1L * nL + SL * (ar1L * ratio - ar1L);
-    f[1] = ar2L * nL + SL * (ar2L * ratio - ar2L);
-    if (DIR == 0) {
-      f[2] = (rhoL * uL * uL + pL) + SL * (rhoL * SP * ratio - rhoL * uL);
-      f[3] = (rhoL * uL * vL) + SL * (rhoL * vL * ratio - rhoL * vL);
-    } else {
-      f[2] = (rhoL * uL * vL) + SL * (rhoL * uL * ratio - rhoL * uL);
-      f[3] = (rhoL * vL * vL + pL) + SL * (rhoL * SP * ratio - rhoL * vL);
-    }
-    f[4] = (EL + pL) * nL + SL * (EL_star * ratio - EL);
-    f[5] = a1L * nL + SL * (a1L * ratio - a1L);
-  } else if (SP <= T(0) && T(0) <= SR) {
-    T ratio = ratR;
-    T ER_star = ER + (SP - nR) * (SP * rhoR + dv(pR, SR - nR));  // SR - nR >= cR > 0
-    f[0] = ar1R * nR + SR * (ar1R * ratio - ar1R);
-    f[1] = ar2R * nR + SR * (ar2R * ratio - ar2R);
-    if (DIR == 0) {
-      f[2] = (rhoR * uR * uR + pR) + SR * (rhoR * SP * ratio - rhoR * uR);
-      f[3] = (rhoR * uR * vR) + SR * (rhoR * vR * ratio - rhoR * vR);
-    } else {
-      f[2] = (rhoR * uR * vR) + SR * (rhoR * uR * ratio - rhoR * uR);
-      f[3] = (rhoR * vR * vR + pR) + SR * (rhoR * SP * ratio - rhoR * vR);
-    }
-    f[4] = (ER + pR) * nR + SR * (ER_star * ratio - ER);
-    f[5] = a1R * nR + SR * (a1R * ratio - a1R);
+  const bool c1 = SL > T(0);
+  const bool c2 = !c1 && SL <= T(0) && T(0) < SP;
+  const bool c3 = !c1 && !c2 && SP <= T(0) && T(0) <= SR;
+  const bool left = c1 || c2, star = c2 || c3;
+  const T ar1 = left ? ar1L : ar1R, ar2 = left ? ar2L : ar2R;
+  const T u = left ? uL : uR, v = left ? vL : vR, p = left ? pL : pR, a1 = left ? a1L : a1R;
+  const T rho = left ? rhoL : rhoR, n = left ? nL : nR, E = left ? EL : ER;
+  const T S = left ? SL : SR, ratio = left ? ratL : ratR;
+  // SL - nL <= -cL < 0 and SR - nR >= cR > 0 wherever the star state is taken
+  const T E_star = E + (SP - n) * (SP * rho + dv(p, S - n));
+  T b[6], c[6];  // the case's flux without and with the star correction
+  b[0] = ar1 * n;
+  b[1] = ar2 * n;
+  if (DIR == 0) {
+    b[2] = rho * u * u + p;
+    b[3] = rho * u * v;
   } else {
-    f[0] = ar1R * nR;
-    f[1] = ar2R * nR;
-    if (DIR == 0) { f[2] = rhoR * uR * uR + pR; f[3] = rhoR * uR * vR; }
-    else          { f[2] = rhoR * uR * vR;      f[3] = rhoR * vR * vR + pR; }
-    f[4] = (ER + pR) * nR;
-    f[5] = a1R * nR;
+    b[2] = rho * u * v;
+    b[3] = rho * v * v + p;
   }
+  b[4] = (E + p) * n;
+  b[5] = a1 * n;
+  c[0] = b[0] + S * (ar1 * ratio - ar1);
+  c[1] = b[1] + S * (ar2 * ratio - ar2);
+  if (DIR == 0) {
+    c[2] = b[2] + S * (rho * SP * ratio - rho * u);
+    c[3] = b[3] + S * (rho * v * ratio - rho * v);
+  } else {
+    c[2] = b[2] + S * (rho * u * ratio - rho * u);
+    c[3] = b[3] + S * (rho * SP * ratio - rho * v);
+  }
+  c[4] = b[4] + S * (E_star * ratio - E);
+  c[5] = b[5] + S * (a1 * ratio - a1);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) f[m] = star ? c[m] : b[m];
   T left_part = nL + wmin(T(0), SL) * (ratL - T(1));
   T right_part = nR + wmax(T(0), SR) * (ratR - T(1));
   T sp_sign = wsign(SP);
   ustar = T(0.5) * (T(1) + sp_sign) * left_part + T(0.5) * (T(1) - sp_sign) * right_part;
+}
+template <int DIR, typename T>
+WAMP_FN void hllc(const Eos& e, const T* L, const T* R, T* f, T& ustar) {
+  const T cL = sound_speed(e, L[0] + L[1], L[4], L[5]);
+  const T cR = sound_speed(e, R[0] + R[1], R[4], R[5]);
+  const T EL = prim_energy(e, L[0], L[1], L[2], L[3], L[4], L[5]);
+  const T ER = prim_energy(e, R[0], R[1], R[2], R[3], R[4], R[5]);
+  hllc_ce<DIR>(e, L, R, cL, cR, EL, ER, f, ustar);
 }
 
 // HLLC for the mixed-precision path: wave speeds, contact speed and the

@@ -221,24 +221,20 @@ WAMP_FN void face_high(const Eos& e, const FaceSmem<TL>& S, int r0, const double
 template <class TL, int DIR, typename T>
 WAMP_FN void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) {
   constexpr int s = DIR == 0 ? TL::RJ : 1;
-  const int rL = r0, rR = r0 + s;
-  double rhoL = S.q[0][rL] + S.q[1][rL];
-  double rhoR = S.q[0][rR] + S.q[1][rR];
-  double uL = dv(S.q[2][rL], rhoL);
-  double vL = dv(S.q[3][rL], rhoL);
-  double uR = dv(S.q[2][rR], rhoR);
-  double vR = dv(S.q[3][rR], rhoR);
-  double aL = clamp01_pp(S.q[5][rL]);
-  double aR = clamp01_pp(S.q[5][rR]);
-  double pL = S.q[4][rL];
-  double pR = S.q[4][rR];
-  double gL, piL, gR, piR;
-  mix_g_pi(e, aL, gL, piL);
-  mix_g_pi(e, aR, gR, piR);
-  if (isnan_(pL) || ddiv(pL + piL, gL - 1.0) < PP_CTILDE) pL = PP_CTILDE * (gL - 1.0) - piL;
-  if (isnan_(pR) || ddiv(pR + piR, gR - 1.0) < PP_CTILDE) pR = PP_CTILDE * (gR - 1.0) - piR;
-  T PL[6] = {(T)S.q[0][rL], (T)S.q[1][rL], (T)uL, (T)vL, (T)pL, (T)aL};
-  T PR[6] = {(T)S.q[0][rR], (T)S.q[1][rR], (T)uR, (T)vR, (T)pR, (T)aR};
+  double pd[2][6];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int r = r0 + k * s;
+    const double rho = S.q[0][r] + S.q[1][r];
+    low_prims(e, S.q[0][r], S.q[1][r], dv(S.q[2][r], rho), dv(S.q[3][r], rho), S.q[4][r],
+              S.q[5][r], pd[k]);
+  }
+  T PL[6], PR[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    PL[m] = (T)pd[0][m];
+    PR[m] = (T)pd[1][m];
+  }
   if constexpr (sizeof(T) == sizeof(double)) {
     hllc<DIR>(e, PL, PR, out, out[6]);
   } else {
@@ -293,7 +289,7 @@ WAMP_FN void tile_sensor_y(FaceSmem<TL>& S, int k) {
 }
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
 WAMP_FN void tile_faces_at(const Geo& G, const Eos& e, const Faces& F, int vf_always, double t1,
-                           int i0, int j0, int t, const FaceSmem<TL>& S) {
+                           int low, int i0, int j0, int t, const FaceSmem<TL>& S) {
   constexpr int FTJ = TL::TJ, FRJ = TL::RJ;
   const int ti = t / FTJ, tj = t % FTJ;
   const int i = i0 + ti, j = j0 + tj;
@@ -309,9 +305,11 @@ WAMP_FN void tile_faces_at(const Geo& G, const Eos& e, const Faces& F, int vf_al
     face_high<TL, 0, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) fstore<T>(F, m, o, out[m]);
-    face_low<TL, 0, T>(e, S, r0, out);
+    if (low) {
+      face_low<TL, 0, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) fstore<T>(F, 14 + m, o, out[m]);
+      for (int m = 0; m < 7; ++m) fstore<T>(F, 14 + m, o, out[m]);
+    }
   }
   if (i >= G.g && i <= G.g + G.nx - 1 && j >= G.g - 1 && j <= G.g + G.ny - 1) {
     double ci[4], sg[4], out[7];
@@ -323,16 +321,18 @@ WAMP_FN void tile_faces_at(const Geo& G, const Eos& e, const Faces& F, int vf_al
     face_high<TL, 1, BASE, SHEAR, FACEPP, T>(e, S, r0, ci, sg, vf_always, t1, out);
 #pragma unroll
     for (int m = 0; m < 7; ++m) fstore<T>(F, 7 + m, o, out[m]);
-    face_low<TL, 1, T>(e, S, r0, out);
+    if (low) {
+      face_low<TL, 1, T>(e, S, r0, out);
 #pragma unroll
-    for (int m = 0; m < 7; ++m) fstore<T>(F, 21 + m, o, out[m]);
+      for (int m = 0; m < 7; ++m) fstore<T>(F, 21 + m, o, out[m]);
+    }
   }
 }
 
 // Tile (bi, bj) by the 256 threads of a block.
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
 __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const State& U, const Faces& F,
-                                           int vf_always, double t1, int bi, int bj,
+                                           int vf_always, double t1, int low, int bi, int bj,
                                            FaceSmem<TL>& S) {
   const int tid = threadIdx.x;
   const int i0 = G.g - 1 + bi * TL::TI;
@@ -342,14 +342,14 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   for (int k = tid; k < TL::CXN; k += FTHREADS) tile_sensor_x<TL, SHEAR>(S, k);
   for (int k = tid; k < TL::CYN; k += FTHREADS) tile_sensor_y<TL, SHEAR>(S, k);
   __syncthreads();
-  tile_faces_at<TL, BASE, SHEAR, FACEPP, T>(G, e, F, vf_always, t1, i0, j0, tid, S);
+  tile_faces_at<TL, BASE, SHEAR, FACEPP, T>(G, e, F, vf_always, t1, low, i0, j0, tid, S);
 }
 
 template <class TL, int BASE, int SHEAR, int FACEPP, typename T>
 __global__ void __launch_bounds__(FTHREADS, (BASE == 0 || sizeof(T) == 4) ? 3 : 2)
-k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1) {
+k_faces(Geo G, Eos e, State U, Faces F, int vf_always, double t1, int low) {
   __shared__ FaceSmem<TL> S;
-  faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, (int)blockIdx.y,
+  faces_tile<TL, BASE, SHEAR, FACEPP, T>(G, e, U, F, vf_always, t1, low, (int)blockIdx.y,
                                          (int)blockIdx.x, S);
 }
 
@@ -378,7 +378,7 @@ k_faces_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ tile_off,
 #pragma unroll
     for (int m = 0; m < 6; ++m) U.f[m] = c.u[stage][m];
     const Faces F = c.f;
-    faces_tile<TL, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, F, vf_always, t1, lt / c.tiles_j,
+    faces_tile<TL, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, F, vf_always, t1, 1, lt / c.tiles_j,
                                            lt % c.tiles_j, S);
     __syncthreads();  // the next tile overwrites the LDS
   }

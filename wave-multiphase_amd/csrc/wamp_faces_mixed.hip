@@ -25,7 +25,7 @@ static void launch_faces_m(const Geo& G, const Eos& e, const State& U, const Fac
   using TL = Tile8x32;
   dim3 grid((G.ny + 1 + TL::TJ - 1) / TL::TJ, (G.nx + 1 + TL::TI - 1) / TL::TI);
   hipLaunchKernelGGL((k_faces<TL, BASE, SHEAR, FACEPP, float>), grid, dim3(FTHREADS), 0, st, G,
-                     e, U, F, vf_always, t1);
+                     e, U, F, vf_always, t1, 1);
 }
 
 }  // namespace mixed

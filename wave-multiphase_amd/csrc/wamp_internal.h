@@ -28,7 +28,8 @@ struct StepPlan {
   int precision;
   int device;
   double t1;     // tanh(beta_thinc / 2), SC:334
-  int lu_kind;   // 0: LDS-tile limiter/update kernel, 1: marching kernel, -1: by grid size
+  int lu_kind;   // 0: LDS-tile limiter/update kernel, 1: marching kernel, 2: march forming the
+                 // first-order fluxes itself (FP64), -1: by grid size
   int lu_seg;    // rows per wave of the marching kernel (0: by grid size)
   int cu_count;  // compute units of the device (limiter segment choice)
 };

@@ -579,6 +579,453 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
 }
 
 // ========================================================================
+// k_lu_lo -- the march of k_lu_march with the first-order fluxes formed in
+// the march (FP64 face storage).  The face kernel then writes only the 14
+// high-order planes: the first-order flux of a face depends on its two cells
+// alone (fx_low_kernel SC:1303-1347, gy_low_kernel SC:1350-1394), and the
+// march streams those cells' state anyway.  Per launch that removes 112 B per
+// cell of face writes and 112 B per cell of face reads, and the first-order
+// HLLC moves out of the face kernel, which is bound by VALU issue, into this
+// one, which is bound by memory.
+//
+// One wave owns 64 consecutive j: 56 output columns and a 4-lane halo each
+// side.  The y-face above lane l is formed from the cells of lanes l and l+1
+// (a lane shift), so lane 63's is not formed; with the output columns at
+// lanes 4..59 nothing an output depends on reaches lanes 0 or 63 (update(l)
+// <- theta2(l+-1) <- theta1(l+-2) <- theta0(l+-3) <- y-faces of lanes
+// l-4 .. l+2).
+//
+// At front f the wave takes row f+1 of the stage input U (at the boundary-
+// mapped cell, slip signs applied, as the face kernel stages it), forms its
+// first-order primitives (low_prims, the face kernel's function) and from
+// them and row f's the first-order fluxes of x-face row f and of the y-faces
+// of row f.  Then, as k_lu_march: theta0(f), theta1(f-1), theta2(f-2), the
+// final fluxes and the update of row f-3.  Carried between fronts:
+//   registers  row f's primitives; first-order x-face rows f-3..f (all
+//              components); high-order x-face rows as in k_lu_march;
+//   LDS ring   first-order y-faces rows f-2..f, high-order y-face {0,1}
+//              rows f-2..f and {5,6} rows f-2..f-1, U rows f-3..f+1.
+// The wave runs alone on its SIMD (one 4-wave block per CU: the ring takes
+// 31 KB of LDS per wave and the carries most of the 512 registers), so the
+// next front's global loads are issued before the current front's
+// arithmetic (fronts in pairs over two load buffers), at in-bounds addresses
+// (rows and columns clamped).  Loaded and formed values of faces outside the
+// domain are used as they come: they feed only thetas of cells outside the
+// interior (which stay 1) and fluxes that no output row or column reads.
+// ========================================================================
+constexpr int LOH = 4;                // halo lanes each side
+constexpr int LOJ = MW - 2 * LOH;     // output columns per wave (56)
+constexpr int LO_YH01 = 0;            // y-face H0 H1, 3 row slots
+constexpr int LO_YH56 = 6;            // y-face H5 H6, 2 row slots
+constexpr int LO_YL = 10;             // y-face first-order L0..L6, 3 row slots
+constexpr int LO_U = 31;              // U0..U5, 5 row slots
+constexpr int LO_CARRY = 61;
+constexpr int LO_WAVES = 4;
+
+// One front's global loads.
+struct LoLoads {
+  double q[6];        // U row f+1 at its boundary-mapped cell
+  double sgu, sgv;    // slip signs of that cell
+  double xh01[2];     // x-face H0 H1 row f
+  double xh56[2];     // x-face H5 H6 row f-1
+  double xh24[3];     // x-face H2 H3 H4 row f-2
+  double yh01[2];     // y-face H0 H1 row f
+  double yh56[2];     // y-face H5 H6 row f-1
+  double yh24[3];     // y-face H2 H3 H4 row f-2
+  double un[6];       // Un row f-3
+};
+
+__device__ __forceinline__ void lo_load(const Geo& G, const State& Un, const State& U,
+                                        const Faces& F, int j, int jc, int f, LoLoads& L) {
+  const size_t ld = (size_t)G.ld;
+  const int last = G.nx + 2 * G.g - 1;
+  auto at = [&](int r) { return (size_t)min(max(r, 0), last) * ld + (size_t)jc; };
+  double sru, srv;
+  const size_t cq = map_cell(G, f + 1, j, sru, srv);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) L.q[m] = U.f[m][cq];
+  L.sgu = sru;
+  L.sgv = srv;
+  const size_t c0 = at(f), c1 = at(f - 1), c2 = at(f - 2), c3 = at(f - 3);
+  L.xh01[0] = F[0][c0];
+  L.xh01[1] = F[1][c0];
+  L.xh56[0] = F[5][c1];
+  L.xh56[1] = F[6][c1];
+  L.yh01[0] = F[7][c0];
+  L.yh01[1] = F[8][c0];
+  L.yh56[0] = F[12][c1];
+  L.yh56[1] = F[13][c1];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    L.xh24[k] = F[2 + k][c2];
+    L.yh24[k] = F[9 + k][c2];
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) L.un[m] = Un.f[m][c3];
+}
+
+// First-order primitives of a staged cell (the conserved values of its
+// boundary-mapped cell, slip signs sgu / sgv) -- pressure recovery as the face
+// kernel's staging (tile_stage_cell), then low_prims -- with the sound speed
+// and total energy that hllc forms from them.  Every face of the cell reuses
+// these: the march forms them once per cell instead of once per face side.
+__device__ __forceinline__ void lo_cell(const Eos& e, const double* q, double sgu, double sgv,
+                                        double* P, double& c, double& E) {
+  const double ru = sgu < 0.0 ? -1.0 * q[2] : q[2];
+  const double rv = sgv < 0.0 ? -1.0 * q[3] : q[3];
+  double uu, vv;
+  const double p = pressure(e, q[0], q[1], ru, rv, q[4], q[5], uu, vv);
+  low_prims(e, q[0], q[1], uu, vv, p, q[5], P);
+  c = sound_speed(e, P[0] + P[1], P[4], P[5]);
+  E = prim_energy(e, P[0], P[1], P[2], P[3], P[4], P[5]);
+}
+
+template <int SEG>
+__device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const State& Un,
+                                             const State& U, const Faces& F, const State& Uo,
+                                             double alpha, double beta, double dt, int repair,
+                                             bool want_dt, int strip, int seg, double* cw) {
+  const int lane = threadIdx.x & (MW - 1);
+  const int j = G.g + strip * LOJ - LOH + lane;  // >= g - 4 >= 0
+  const int i0 = G.g + seg * SEG;
+  if (i0 >= G.g + G.nx) return 1.0e30;  // wave-uniform
+  const int i_end = min(i0 + SEG, G.g + G.nx);
+  const double inv_dx = 1.0 / G.dx;
+  const double inv_dy = 1.0 / G.dy;
+  const int jc = min(j, G.ld - 1);
+  const bool col_in = j >= G.g && j < G.g + G.ny;
+  const bool yface_in = j >= G.g - 1 && j < G.g + G.ny;
+  const bool out_lane = lane >= LOH && lane < LOH + LOJ && col_in;
+  auto row_in = [&](int r) { return r >= G.g && r < G.g + G.nx; };
+  auto xrow_in = [&](int r) { return r >= G.g - 1 && r < G.g + G.nx; };
+  auto lds = [&](int q) -> double& { return cw[q * MW + lane]; };
+
+#pragma unroll
+  for (int q = 0; q < LO_CARRY; ++q) lds(q) = 0.0;
+
+  double P0[6];               // first-order primitives of row f
+  double Lx[4][7];            // first-order x-face fluxes rows f-3 .. f
+  double xh01[3][2] = {};     // high-order x-face H0 H1 rows f-2 .. f
+  double xh56[2][2] = {};     // H5 H6 rows f-2, f-1
+  double xa[2][7] = {};       // H0..H6 rows f-3, f-2
+  double F2xm[7] = {};        // final flux at x-face row f-4 (below the update row)
+  double F2yu[7] = {};        // final flux at the y-faces above row f-3
+  double t0[5] = {1.0, 1.0, 1.0, 1.0, 1.0};  // theta0 rows f-4 .. f
+  double t1[4] = {1.0, 1.0, 1.0, 1.0};       // theta1 rows f-4 .. f-1
+  double t2[3] = {1.0, 1.0, 1.0};            // theta2 rows f-4 .. f-2
+  double local_dt = 1.0e30;
+
+  // Before the first front (f = i0-3): row i0-3's primitives and U (theta0 of
+  // row i0-3 reads its U0, U1 from the ring), and x-face row i0-4 below it --
+  // theta0(i0-3) is in the first output row's dependency cone.
+  double c0, E0;  // row f's sound speed and total energy
+  {
+    double q[6], Pm[6], cm, Em;
+    LoLoads a;
+    lo_load(G, Un, U, F, j, jc, i0 - 5, a);  // U row i0-4 (faces unused)
+    lo_cell(e, a.q, a.sgu, a.sgv, Pm, cm, Em);
+    double sru, srv;
+    const size_t cq = map_cell(G, i0 - 3, j, sru, srv);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) q[m] = U.f[m][cq];
+    lo_cell(e, q, sru, srv, P0, c0, E0);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) lds(LO_U + 6 * ((i0 - 3) % 5) + m) = q[m];
+    double lx[7];
+    hllc_ce<0>(e, Pm, P0, cm, c0, Em, E0, lx, lx[6]);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      Lx[0][m] = 0.0;
+      Lx[1][m] = 0.0;
+      Lx[2][m] = 0.0;
+      Lx[3][m] = lx[m];
+    }
+    const size_t c = (size_t)(i0 - 4) * (size_t)G.ld + (size_t)jc;  // i0 - 4 >= 0
+    xh01[2][0] = F[0][c];
+    xh01[2][1] = F[1][c];
+  }
+
+  // ring slots of rows f (U ring: row mod 5; y rings: row mod 3), advanced per
+  // front instead of divided (f >= i0 - 3 >= 1)
+  int s5f = (i0 - 3) % 5, s3f = (i0 - 3) % 3;
+  // One front; `cur` holds its global loads (issued one front earlier).
+  auto front = [&](const int f, const LoLoads& cur) __attribute__((always_inline)) {
+    const int s5p1 = s5f == 4 ? 0 : s5f + 1, s5m1 = s5f == 0 ? 4 : s5f - 1;
+    const int s5m2 = s5m1 == 0 ? 4 : s5m1 - 1, s5m3 = s5m2 == 0 ? 4 : s5m2 - 1;
+    const int s3m1 = s3f == 0 ? 2 : s3f - 1, s3m2 = s3m1 == 0 ? 2 : s3m1 - 1;
+    const int s2m1 = (f + 1) & 1, s2m2 = f & 1;
+
+    // ---- row f+1's primitives; first-order fluxes of x-face row f and of
+    //      the y-faces of row f
+    double ly[7];
+    {
+      double P1[6], lx[7], Pd[6], c1, E1;
+      lo_cell(e, cur.q, cur.sgu, cur.sgv, P1, c1, E1);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) lds(LO_U + 6 * s5p1 + m) = cur.q[m];
+      hllc_ce<0>(e, P0, P1, c0, c1, E0, E1, lx, lx[6]);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) Pd[m] = lane_down(P0[m]);
+      hllc_ce<1>(e, P0, Pd, c0, lane_down(c0), E0, lane_down(E0), ly, ly[6]);
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        Lx[0][m] = Lx[1][m];
+        Lx[1][m] = Lx[2][m];
+        Lx[2][m] = Lx[3][m];
+        Lx[3][m] = lx[m];
+        lds(LO_YL + 7 * s3f + m) = ly[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) P0[m] = P1[m];
+      c0 = c1;
+      E0 = E1;
+    }
+
+    // ---- high-order x-faces of this front
+    {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        xh01[0][q] = xh01[1][q];
+        xh01[1][q] = xh01[2][q];
+        xh01[2][q] = cur.xh01[q];
+        xh56[0][q] = xh56[1][q];
+        xh56[1][q] = cur.xh56[q];
+      }
+#pragma unroll
+      for (int m = 0; m < 7; ++m) xa[0][m] = xa[1][m];
+      xa[1][0] = xh01[0][0];
+      xa[1][1] = xh01[0][1];
+      xa[1][5] = xh56[0][0];
+      xa[1][6] = xh56[0][1];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xa[1][2 + k] = cur.xh24[k];
+    }
+
+    // ---------------- theta0 at row f (raw high-order fluxes) ----------------
+    {
+      const bool ok = row_in(f) && col_in;
+      double yc[2], ycd[2], lyd[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        yc[m] = cur.yh01[m];
+        ycd[m] = lane_up(yc[m]);
+        lyd[m] = lane_up(ly[m]);
+      }
+      double th = 1.0;
+      if (ok) {
+        double l[6], h[6];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          const double u = lds(LO_U + 6 * s5f + m);
+          double rh = (xh01[2][m] - xh01[1][m]) * inv_dx + (yc[m] - ycd[m]) * inv_dy;
+          double rl = (Lx[3][m] - Lx[2][m]) * inv_dx + (ly[m] - lyd[m]) * inv_dy;
+          l[m] = u - dt * rl;
+          h[m] = u - dt * rh;
+        }
+        th = theta_stage<0>(e, l, h);
+      }
+      lds(LO_YH01 + 2 * s3f) = yc[0];
+      lds(LO_YH01 + 2 * s3f + 1) = yc[1];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t0[k] = t0[k + 1];
+      t0[4] = th;
+    }
+
+    // ---------------- theta1 at row f-1 (fluxes blended once) ----------------
+    {
+      const int r = f - 1;
+      const bool ok = row_in(r) && col_in;
+      double tf[4];
+      tf[0] = face_theta(t0[2], t0[3]);
+      tf[1] = face_theta(t0[3], t0[4]);
+      tf[2] = face_theta(lane_up(t0[3]), t0[3]);
+      tf[3] = face_theta(t0[3], lane_down(t0[3]));
+      double yu[2], yul[2], yd[2], ydl[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        yu[q] = cur.yh56[q];
+        yul[q] = lds(LO_YL + 7 * s3m1 + 5 + q);
+        yd[q] = lane_up(yu[q]);
+        ydl[q] = lane_up(yul[q]);
+      }
+      double th1 = 1.0;
+      if (ok) {
+        const double al = lds(LO_U + 6 * s5m1 + 5);
+        double Hxp5 = blend3(xh56[1][0], Lx[2][5], &tf[1], 1);
+        double Hxm5 = blend3(xh56[0][0], Lx[1][5], &tf[0], 1);
+        double Hyp5 = blend3(yu[0], yul[0], &tf[3], 1);
+        double Hym5 = blend3(yd[0], ydl[0], &tf[2], 1);
+        double Hxp6 = blend3(xh56[1][1], Lx[2][6], &tf[1], 1);
+        double Hxm6 = blend3(xh56[0][1], Lx[1][6], &tf[0], 1);
+        double Hyp6 = blend3(yu[1], yul[1], &tf[3], 1);
+        double Hym6 = blend3(yd[1], ydl[1], &tf[2], 1);
+        double rh = (Hxp5 - Hxm5) * inv_dx + (Hyp5 - Hym5) * inv_dy;
+        double rl = (Lx[2][5] - Lx[1][5]) * inv_dx + (yul[0] - ydl[0]) * inv_dy;
+        rh = rh - al * (Hxp6 - Hxm6) * inv_dx - al * (Hyp6 - Hym6) * inv_dy;
+        rl = rl - al * (Lx[2][6] - Lx[1][6]) * inv_dx - al * (yul[1] - ydl[1]) * inv_dy;
+        double l[6], h[6];
+        l[5] = al - dt * rl;
+        h[5] = al - dt * rh;
+        th1 = theta_stage<1>(e, l, h);
+      }
+      lds(LO_YH56 + 2 * s2m1) = yu[0];
+      lds(LO_YH56 + 2 * s2m1 + 1) = yu[1];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t1[k] = t1[k + 1];
+      t1[3] = th1;
+    }
+
+    // ---------------- theta2 at row f-2 (fluxes blended twice) ----------------
+    double yn[14];  // y-faces above row f-2 (H0..6, L0..6)
+    {
+      const int r = f - 2;
+      const bool ok = row_in(r) && col_in;
+      yn[0] = lds(LO_YH01 + 2 * s3m2);
+      yn[1] = lds(LO_YH01 + 2 * s3m2 + 1);
+      yn[5] = lds(LO_YH56 + 2 * s2m2);
+      yn[6] = lds(LO_YH56 + 2 * s2m2 + 1);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) yn[2 + k] = cur.yh24[k];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) yn[7 + m] = lds(LO_YL + 7 * s3m2 + m);
+      double tf[4][2];
+      tf[0][0] = face_theta(t0[1], t0[2]);
+      tf[1][0] = face_theta(t0[2], t0[3]);
+      tf[2][0] = face_theta(lane_up(t0[2]), t0[2]);
+      tf[3][0] = face_theta(t0[2], lane_down(t0[2]));
+      tf[0][1] = face_theta(t1[1], t1[2]);
+      tf[1][1] = face_theta(t1[2], t1[3]);
+      tf[2][1] = face_theta(lane_up(t1[2]), t1[2]);
+      tf[3][1] = face_theta(t1[2], lane_down(t1[2]));
+      double ydn[14];
+#pragma unroll
+      for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
+      double th2 = 1.0;
+      if (ok) {
+        double uc[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) uc[m] = lds(LO_U + 6 * s5m2 + m);
+        const double al = uc[5];
+        // component 6 (contact velocity) first: it enters component 5's residual
+        const double s_hm = blend3(xa[0][6], Lx[0][6], tf[0], 2);
+        const double s_hp = blend3(xa[1][6], Lx[1][6], tf[1], 2);
+        const double s_hd = blend3(ydn[6], ydn[13], tf[2], 2);
+        const double s_hu = blend3(yn[6], yn[13], tf[3], 2);
+        double l[6], h[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double hm = blend3(xa[0][m], Lx[0][m], tf[0], 2);
+          double hp = blend3(xa[1][m], Lx[1][m], tf[1], 2);
+          double hd = blend3(ydn[m], ydn[7 + m], tf[2], 2);
+          double hu = blend3(yn[m], yn[7 + m], tf[3], 2);
+          double rh = (hp - hm) * inv_dx + (hu - hd) * inv_dy;
+          double rl = (Lx[1][m] - Lx[0][m]) * inv_dx + (yn[7 + m] - ydn[7 + m]) * inv_dy;
+          if (m == 5) {
+            rh = rh - al * (s_hp - s_hm) * inv_dx - al * (s_hu - s_hd) * inv_dy;
+            rl = rl - al * (Lx[1][6] - Lx[0][6]) * inv_dx - al * (yn[13] - ydn[13]) * inv_dy;
+          }
+          const double u = uc[m];
+          l[m] = u - dt * rl;
+          h[m] = u - dt * rh;
+        }
+        th2 = theta_stage<2>(e, l, h);
+      }
+#pragma unroll
+      for (int k = 0; k < 2; ++k) t2[k] = t2[k + 1];
+      t2[2] = th2;
+    }
+
+    // ---- final fluxes (three blends): x-face row f-3, y-faces above row f-2
+    double F2xp[7], F2yn[7];
+    {
+      double tx[3], ty[3];
+      const double* tt[3] = {t0 + 1, t1 + 1, t2 + 1};  // rows f-3 at [0], f-2 at [1]
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        tx[k] = face_theta(tt[k][0], tt[k][1]);
+        ty[k] = face_theta(tt[k][1], lane_down(tt[k][1]));
+      }
+#pragma unroll
+      for (int m = 0; m < 7; ++m) {
+        F2xp[m] = blend3(xa[0][m], Lx[0][m], tx, 3);
+        F2yn[m] = blend3(yn[m], yn[7 + m], ty, 3);
+      }
+    }
+
+    // ---------------- update of row f-3 ----------------
+    {
+      const int r = f - 3;
+      const bool upd = r >= i0 && r < i_end && out_lane;
+      double yd[7];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) yd[m] = lane_up(F2yu[m]);
+      if (upd) {
+        const size_t c = (size_t)r * (size_t)G.ld + (size_t)j;
+        double uc[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) uc[m] = lds(LO_U + 6 * s5m3 + m);
+        const double ap = uc[5];
+        double un[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double res = (F2xp[m] - F2xm[m]) * inv_dx + (F2yu[m] - yd[m]) * inv_dy;
+          if (m == 5)
+            res = res - ap * (F2xp[6] - F2xm[6]) * inv_dx - ap * (F2yu[6] - yd[6]) * inv_dy;
+          un[m] = alpha * cur.un[m] + beta * (uc[m] - dt * res);
+        }
+        if (repair) repair_cell(e, un);
+        un[5] = clamp_al(un[5]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) Uo.f[m][c] = un[m];
+        if (want_dt) local_dt = wmin(local_dt, cell_dt(e, un, G.dx, G.dy));
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      F2xm[m] = F2xp[m];
+      F2yu[m] = F2yn[m];
+    }
+    s5f = s5p1;
+    s3f = s3f == 2 ? 0 : s3f + 1;
+  };
+
+  // Fronts i0-3 .. i_end+2, run in pairs over two load buffers that swap
+  // roles, so a front's loads are in flight during the previous front's
+  // arithmetic and no register copy waits for them.  An odd count gets one
+  // more front after the last: it updates no row (row i_end is past the
+  // segment) and its loads stay in bounds.
+  LoLoads A, B;
+  lo_load(G, Un, U, F, j, jc, i0 - 3, A);
+  const int f_end = i0 - 3 + (((i_end - i0 + 6) + 1) & ~1);
+  for (int f = i0 - 3; f < f_end; f += 2) {
+    lo_load(G, Un, U, F, j, jc, f + 1, B);
+    front(f, A);
+    lo_load(G, Un, U, F, j, jc, f + 2, A);
+    front(f + 1, B);
+  }
+  return local_dt;
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(64 * LO_WAVES, 1)
+k_lu_lo(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
+        const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+  __shared__ double carry[LO_WAVES * LO_CARRY * MW];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
+  // the XCD-aware block map of k_lu_march (a bijection on [0, N))
+  const int nblk = (int)(gridDim.x * gridDim.y);
+  const int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  const int xcd = lin & 7, base = nblk >> 3, extra = nblk & 7;
+  const int q = xcd * base + (xcd < extra ? xcd : extra) + (lin >> 3);
+  const int strip = q % (int)gridDim.x;
+  const int seg = (q / (int)gridDim.x) * LO_WAVES + wave;
+  if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
+  double v = lu_wave_lo<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
+                             strip, seg, carry + wave * (LO_CARRY * MW));
+  if (dt_acc) wave_min_atomic(v, dt_acc);
+}
+
+// ========================================================================
 // arithmetic self-test: the FP64 kernels' division and square-root
 // sequences beside the compiler's IEEE operations, element by element
 // ========================================================================
@@ -877,15 +1324,13 @@ __global__ void __launch_bounds__(256) k_ic_triple(Geo G, Eos e, State U, Triple
 // ========================================================================
 // launch helpers (called by the C-ABI layer)
 // ========================================================================
-typedef void (*faces_fn)(Geo, Eos, State, Faces, int, double);
-
 template <int BASE, int SHEAR, int FACEPP, typename T>
 static void launch_faces_t(const Geo& G, const Eos& e, const State& U, const Faces& F,
-                           int vf_always, double t1, hipStream_t st) {
+                           int vf_always, double t1, int low, hipStream_t st) {
   using TL = Tile8x32;
   dim3 grid((G.ny + 1 + TL::TJ - 1) / TL::TJ, (G.nx + 1 + TL::TI - 1) / TL::TI);
   hipLaunchKernelGGL((k_faces<TL, BASE, SHEAR, FACEPP, T>), grid, dim3(FTHREADS), 0, st, G, e, U,
-                     F, vf_always, t1);
+                     F, vf_always, t1, low);
 }
 
 }  // namespace wamp
@@ -903,15 +1348,17 @@ hipError_t wamp_launch_faces_batch_mixed(const CaseDesc* cs, const int* tile_off
                                          int face_pp, int vf_always, double t1, int tile,
                                          int blocks, hipStream_t st);
 
+// low = 0: the first-order planes are not written (the limiter march that
+// forms the first-order fluxes itself reads only the 14 high-order planes).
 hipError_t wamp_launch_faces(const Geo* G, const Eos* e, const State* U, const Faces* F,
                              int base, int shear, int face_pp, int vf_always, double t1,
-                             int precision, hipStream_t st) {
+                             int precision, int low, hipStream_t st) {
   if (precision != 0)
     return wamp_launch_faces_mixed(G, e, U, F, base, shear, face_pp, vf_always, t1, st);
-#define WAMP_F(B, S, P)                                                      \
-  if (base == B && shear == S && face_pp == P) {                            \
-    launch_faces_t<B, S, P, double>(*G, *e, *U, *F, vf_always, t1, st);     \
-    return hipGetLastError();                                               \
+#define WAMP_F(B, S, P)                                                         \
+  if (base == B && shear == S && face_pp == P) {                               \
+    launch_faces_t<B, S, P, double>(*G, *e, *U, *F, vf_always, t1, low, st);   \
+    return hipGetLastError();                                                  \
   }
   WAMP_F(0, 0, 0) WAMP_F(0, 0, 1) WAMP_F(0, 1, 0) WAMP_F(0, 1, 1) WAMP_F(0, 2, 0) WAMP_F(0, 2, 1)
   WAMP_F(1, 0, 0) WAMP_F(1, 0, 1) WAMP_F(1, 1, 0) WAMP_F(1, 1, 1) WAMP_F(1, 2, 0) WAMP_F(1, 2, 1)
@@ -951,6 +1398,26 @@ hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, con
     return hipGetLastError();                                                                   \
   }
   WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(20) WAMP_SEG(26) WAMP_SEG(28) WAMP_SEG(32) WAMP_SEG(64)
+#undef WAMP_SEG
+  return hipErrorInvalidValue;
+}
+
+// The march that forms the first-order fluxes itself (FP64 face storage; the
+// first-order planes of F are not read).
+hipError_t wamp_launch_lu_lo(const Geo* G, const Eos* e, const State* Un, const State* U,
+                             const Faces* F, const State* Uo, double alpha, double beta,
+                             const double* dt_dev, double* dt_acc, int repair, int seg,
+                             hipStream_t st) {
+  if (F->h) return hipErrorInvalidValue;
+  const int nseg = (G->nx + seg - 1) / seg;
+  dim3 grid((G->ny + LOJ - 1) / LOJ, (nseg + LO_WAVES - 1) / LO_WAVES);
+#define WAMP_SEG(S)                                                                           \
+  if (seg == S) {                                                                             \
+    hipLaunchKernelGGL((k_lu_lo<S>), grid, dim3(64 * LO_WAVES), 0, st, *G, *e, *Un, *U, *F,    \
+                       *Uo, alpha, beta, dt_dev, dt_acc, repair);                             \
+    return hipGetLastError();                                                                 \
+  }
+  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(26) WAMP_SEG(32) WAMP_SEG(48) WAMP_SEG(64)
 #undef WAMP_SEG
   return hipErrorInvalidValue;
 }
