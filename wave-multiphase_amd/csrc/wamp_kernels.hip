@@ -165,14 +165,16 @@ constexpr int LU_WAVES = 4;  // waves per limiter block
 // Neighbouring lane's value (lane_up: lane i gets lane i-1; lane_down: lane i
 // gets lane i+1) by a DPP wave shift of each 32-bit half: one VALU move per
 // half instead of an LDS permute.  Lane 0 (lane_up) and lane 63 (lane_down)
-// get 0; they are halo lanes, whose thetas never reach an output lane.
+// get 0 (the move's bound control writes zero where the source lane is out
+// of range, so no zero has to be placed in the destination first); they are
+// halo lanes, whose thetas never reach an output lane.
 constexpr int DPP_WAVE_SHL1 = 0x130;
 constexpr int DPP_WAVE_SHR1 = 0x138;
 template <int CTRL>
 __device__ __forceinline__ double lane_shift(double v) {
   const unsigned long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ double lane_up(double v) { return lane_shift<DPP_WAVE_SHR1>(v); }
@@ -233,6 +235,13 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
   double x56[2][4] = {};   // x-face rows f-2, f-1: H5 H6 L5 L6
   double xa[2][14] = {};   // x-face rows f-3, f-2: H0..6 L0..6
   double F2xm[7] = {};     // final flux at x-face row f-4 (below the update row)
+  // A face's blended flux is the same for the cells on either side of it:
+  // the x-face below theta1's / theta2's row is the face above the previous
+  // front's row (the same face thetas), and a y-face below a lane is the face
+  // above the lane before (a lane shift).  Those are carried or shifted, not
+  // blended twice.
+  double B1x[2] = {};      // x-face row f-2, H5 H6 blended once
+  double B2x[7] = {};      // x-face row f-3, H0..H6 blended twice
   double F2yu[7] = {};     // final flux at the y-faces above row f-3
   double t0[5] = {1.0, 1.0, 1.0, 1.0, 1.0};  // theta0 rows f-4 .. f
   double t1[4] = {1.0, 1.0, 1.0, 1.0};       // theta1 rows f-4 .. f-1
@@ -384,19 +393,20 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       tf[1] = face_theta(t0[3], t0[4]);
       tf[2] = face_theta(lane_up(t0[3]), t0[3]);
       tf[3] = face_theta(t0[3], lane_down(t0[3]));
-      double yd[4];
+      double yd[4];  // [2], [3]: L5 L6 of the face below
 #pragma unroll
-      for (int q = 0; q < 4; ++q) yd[q] = lane_up(yu[q]);
+      for (int q = 2; q < 4; ++q) yd[q] = lane_up(yu[q]);
+      // the faces above and to the side of the row, blended once on every lane
+      const double Hxp5 = blend3(x56[1][0], x56[1][2], &tf[1], 1);
+      const double Hxp6 = blend3(x56[1][1], x56[1][3], &tf[1], 1);
+      const double Hyp5 = blend3(yu[0], yu[2], &tf[3], 1);
+      const double Hyp6 = blend3(yu[1], yu[3], &tf[3], 1);
+      const double Hxm5 = B1x[0], Hxm6 = B1x[1];
+      const double Hym5 = lane_up(Hyp5), Hym6 = lane_up(Hyp6);
+      B1x[0] = Hxp5;
+      B1x[1] = Hxp6;
       double th1 = 1.0;
       if (ok) {
-        double Hxp5 = blend3(x56[1][0], x56[1][2], &tf[1], 1);
-        double Hxm5 = blend3(x56[0][0], x56[0][2], &tf[0], 1);
-        double Hyp5 = blend3(yu[0], yu[2], &tf[3], 1);
-        double Hym5 = blend3(yd[0], yd[2], &tf[2], 1);
-        double Hxp6 = blend3(x56[1][1], x56[1][3], &tf[1], 1);
-        double Hxm6 = blend3(x56[0][1], x56[0][3], &tf[0], 1);
-        double Hyp6 = blend3(yu[1], yu[3], &tf[3], 1);
-        double Hym6 = blend3(yd[1], yd[3], &tf[2], 1);
         double rh = (Hxp5 - Hxm5) * inv_dx + (Hyp5 - Hym5) * inv_dy;
         double rl = (x56[1][2] - x56[0][2]) * inv_dx + (yu[2] - yd[2]) * inv_dy;
         rh = rh - al * (Hxp6 - Hxm6) * inv_dx - al * (Hyp6 - Hym6) * inv_dy;
@@ -416,6 +426,7 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 
     // ---------------- theta2 at row f-2 (fluxes blended twice) ----------------
     double yn[14];  // y-faces above row f-2 (H0..6, L0..6)
+    double hm[7], hu[7];  // x-face row f-3, y-faces above row f-2: blended twice
     {
       const int r = f - 2;
       const bool ok = row_in(r) && col_in;
@@ -441,9 +452,17 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
       tf[1][1] = face_theta(t1[2], t1[3]);
       tf[2][1] = face_theta(lane_up(t1[2]), t1[2]);
       tf[3][1] = face_theta(t1[2], lane_down(t1[2]));
-      double ydn[14];
+      // the faces above and to the side of the row, blended twice on every lane
+      double hp[7], hd[7], ydl[7];
 #pragma unroll
-      for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
+      for (int m = 0; m < 7; ++m) {
+        hp[m] = blend3(xa[1][m], xa[1][7 + m], tf[1], 2);
+        hu[m] = blend3(yn[m], yn[7 + m], tf[3], 2);
+        hm[m] = B2x[m];
+        hd[m] = lane_up(hu[m]);
+        ydl[m] = lane_up(yn[7 + m]);
+        B2x[m] = hp[m];
+      }
       double th2 = 1.0;
       if (ok) {
         double uc[6];
@@ -454,23 +473,14 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
         uc[4] = u234[2];
         uc[5] = lds(LC_U5 + s3m2);
         const double al = uc[5];
-        // component 6 (contact velocity) first: it enters component 5's residual
-        const double s_hm = blend3(xa[0][6], xa[0][13], tf[0], 2);
-        const double s_hp = blend3(xa[1][6], xa[1][13], tf[1], 2);
-        const double s_hd = blend3(ydn[6], ydn[13], tf[2], 2);
-        const double s_hu = blend3(yn[6], yn[13], tf[3], 2);
         double l[6], h[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-          double hm = blend3(xa[0][m], xa[0][7 + m], tf[0], 2);
-          double hp = blend3(xa[1][m], xa[1][7 + m], tf[1], 2);
-          double hd = blend3(ydn[m], ydn[7 + m], tf[2], 2);
-          double hu = blend3(yn[m], yn[7 + m], tf[3], 2);
-          double rh = (hp - hm) * inv_dx + (hu - hd) * inv_dy;
-          double rl = (xa[1][7 + m] - xa[0][7 + m]) * inv_dx + (yn[7 + m] - ydn[7 + m]) * inv_dy;
+          double rh = (hp[m] - hm[m]) * inv_dx + (hu[m] - hd[m]) * inv_dy;
+          double rl = (xa[1][7 + m] - xa[0][7 + m]) * inv_dx + (yn[7 + m] - ydl[m]) * inv_dy;
           if (m == 5) {
-            rh = rh - al * (s_hp - s_hm) * inv_dx - al * (s_hu - s_hd) * inv_dy;
-            rl = rl - al * (xa[1][13] - xa[0][13]) * inv_dx - al * (yn[13] - ydn[13]) * inv_dy;
+            rh = rh - al * (hp[6] - hm[6]) * inv_dx - al * (hu[6] - hd[6]) * inv_dy;
+            rl = rl - al * (xa[1][13] - xa[0][13]) * inv_dx - al * (yn[13] - ydl[6]) * inv_dy;
           }
           const double u = uc[m];
           l[m] = u - dt * rl;
@@ -487,19 +497,15 @@ __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const Stat
 
     // ---- final fluxes (three blends) of the faces whose thetas are now complete:
     //      x-face row f-3 (between rows f-3, f-2) and the y-faces above row f-2
+    //      (their first two blends are theta2's: one more with the stage-2 face theta)
     double F2xp[7], F2yn[7];
     {
-      double tx[3], ty[3];
-      const double* tt[3] = {t0 + 1, t1 + 1, t2 + 1};  // rows f-3 at [0], f-2 at [1]
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        tx[k] = face_theta(tt[k][0], tt[k][1]);
-        ty[k] = face_theta(tt[k][1], lane_down(tt[k][1]));
-      }
+      const double tx = face_theta(t2[1], t2[2]);             // rows f-3, f-2
+      const double ty = face_theta(t2[2], lane_down(t2[2]));  // row f-2 and the lane after
 #pragma unroll
       for (int m = 0; m < 7; ++m) {
-        F2xp[m] = blend3(xa[0][m], xa[0][7 + m], tx, 3);
-        F2yn[m] = blend3(yn[m], yn[7 + m], ty, 3);
+        F2xp[m] = blend3(hm[m], xa[0][7 + m], &tx, 1);
+        F2yn[m] = blend3(hu[m], yn[7 + m], &ty, 1);
       }
     }
 
@@ -709,6 +715,13 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
   double xh56[2][2] = {};     // H5 H6 rows f-2, f-1
   double xa[2][7] = {};       // H0..H6 rows f-3, f-2
   double F2xm[7] = {};        // final flux at x-face row f-4 (below the update row)
+  // A face's blended flux is the same for the cells on either side of it:
+  // the x-face below theta1's / theta2's row is the face above the previous
+  // front's row (the same face thetas), and a y-face below a lane is the face
+  // above the lane before (a lane shift).  Those are carried or shifted, not
+  // blended twice.
+  double B1x[2] = {};         // x-face row f-2, H5 H6 blended once
+  double B2x[7] = {};         // x-face row f-3, H0..H6 blended twice
   double F2yu[7] = {};        // final flux at the y-faces above row f-3
   double t0[5] = {1.0, 1.0, 1.0, 1.0, 1.0};  // theta0 rows f-4 .. f
   double t1[4] = {1.0, 1.0, 1.0, 1.0};       // theta1 rows f-4 .. f-1
@@ -840,25 +853,25 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
       tf[1] = face_theta(t0[3], t0[4]);
       tf[2] = face_theta(lane_up(t0[3]), t0[3]);
       tf[3] = face_theta(t0[3], lane_down(t0[3]));
-      double yu[2], yul[2], yd[2], ydl[2];
+      double yu[2], yul[2], ydl[2];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         yu[q] = cur.yh56[q];
         yul[q] = lds(LO_YL + 7 * s3m1 + 5 + q);
-        yd[q] = lane_up(yu[q]);
         ydl[q] = lane_up(yul[q]);
       }
+      // the faces above and to the side of the row, blended once on every lane
+      const double Hxp5 = blend3(xh56[1][0], Lx[2][5], &tf[1], 1);
+      const double Hxp6 = blend3(xh56[1][1], Lx[2][6], &tf[1], 1);
+      const double Hyp5 = blend3(yu[0], yul[0], &tf[3], 1);
+      const double Hyp6 = blend3(yu[1], yul[1], &tf[3], 1);
+      const double Hxm5 = B1x[0], Hxm6 = B1x[1];
+      const double Hym5 = lane_up(Hyp5), Hym6 = lane_up(Hyp6);
+      B1x[0] = Hxp5;
+      B1x[1] = Hxp6;
       double th1 = 1.0;
       if (ok) {
         const double al = lds(LO_U + 6 * s5m1 + 5);
-        double Hxp5 = blend3(xh56[1][0], Lx[2][5], &tf[1], 1);
-        double Hxm5 = blend3(xh56[0][0], Lx[1][5], &tf[0], 1);
-        double Hyp5 = blend3(yu[0], yul[0], &tf[3], 1);
-        double Hym5 = blend3(yd[0], ydl[0], &tf[2], 1);
-        double Hxp6 = blend3(xh56[1][1], Lx[2][6], &tf[1], 1);
-        double Hxm6 = blend3(xh56[0][1], Lx[1][6], &tf[0], 1);
-        double Hyp6 = blend3(yu[1], yul[1], &tf[3], 1);
-        double Hym6 = blend3(yd[1], ydl[1], &tf[2], 1);
         double rh = (Hxp5 - Hxm5) * inv_dx + (Hyp5 - Hym5) * inv_dy;
         double rl = (Lx[2][5] - Lx[1][5]) * inv_dx + (yul[0] - ydl[0]) * inv_dy;
         rh = rh - al * (Hxp6 - Hxm6) * inv_dx - al * (Hyp6 - Hym6) * inv_dy;
@@ -877,6 +890,7 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
 
     // ---------------- theta2 at row f-2 (fluxes blended twice) ----------------
     double yn[14];  // y-faces above row f-2 (H0..6, L0..6)
+    double hm[7], hu[7];  // x-face row f-3, y-faces above row f-2: blended twice
     {
       const int r = f - 2;
       const bool ok = row_in(r) && col_in;
@@ -897,32 +911,31 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
       tf[1][1] = face_theta(t1[2], t1[3]);
       tf[2][1] = face_theta(lane_up(t1[2]), t1[2]);
       tf[3][1] = face_theta(t1[2], lane_down(t1[2]));
-      double ydn[14];
+      // the faces above and to the side of the row, blended twice on every lane
+      double hp[7], hd[7], ydl[7];
 #pragma unroll
-      for (int m = 0; m < 14; ++m) ydn[m] = lane_up(yn[m]);
+      for (int m = 0; m < 7; ++m) {
+        hp[m] = blend3(xa[1][m], Lx[1][m], tf[1], 2);
+        hu[m] = blend3(yn[m], yn[7 + m], tf[3], 2);
+        hm[m] = B2x[m];
+        hd[m] = lane_up(hu[m]);
+        ydl[m] = lane_up(yn[7 + m]);
+        B2x[m] = hp[m];
+      }
       double th2 = 1.0;
       if (ok) {
         double uc[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) uc[m] = lds(LO_U + 6 * s5m2 + m);
         const double al = uc[5];
-        // component 6 (contact velocity) first: it enters component 5's residual
-        const double s_hm = blend3(xa[0][6], Lx[0][6], tf[0], 2);
-        const double s_hp = blend3(xa[1][6], Lx[1][6], tf[1], 2);
-        const double s_hd = blend3(ydn[6], ydn[13], tf[2], 2);
-        const double s_hu = blend3(yn[6], yn[13], tf[3], 2);
         double l[6], h[6];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-          double hm = blend3(xa[0][m], Lx[0][m], tf[0], 2);
-          double hp = blend3(xa[1][m], Lx[1][m], tf[1], 2);
-          double hd = blend3(ydn[m], ydn[7 + m], tf[2], 2);
-          double hu = blend3(yn[m], yn[7 + m], tf[3], 2);
-          double rh = (hp - hm) * inv_dx + (hu - hd) * inv_dy;
-          double rl = (Lx[1][m] - Lx[0][m]) * inv_dx + (yn[7 + m] - ydn[7 + m]) * inv_dy;
+          double rh = (hp[m] - hm[m]) * inv_dx + (hu[m] - hd[m]) * inv_dy;
+          double rl = (Lx[1][m] - Lx[0][m]) * inv_dx + (yn[7 + m] - ydl[m]) * inv_dy;
           if (m == 5) {
-            rh = rh - al * (s_hp - s_hm) * inv_dx - al * (s_hu - s_hd) * inv_dy;
-            rl = rl - al * (Lx[1][6] - Lx[0][6]) * inv_dx - al * (yn[13] - ydn[13]) * inv_dy;
+            rh = rh - al * (hp[6] - hm[6]) * inv_dx - al * (hu[6] - hd[6]) * inv_dy;
+            rl = rl - al * (Lx[1][6] - Lx[0][6]) * inv_dx - al * (yn[13] - ydl[6]) * inv_dy;
           }
           const double u = uc[m];
           l[m] = u - dt * rl;
@@ -935,20 +948,17 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
       t2[2] = th2;
     }
 
-    // ---- final fluxes (three blends): x-face row f-3, y-faces above row f-2
+    // ---- final fluxes (three blends): x-face row f-3, y-faces above row f-2.
+    //      Their first two blends are theta2's (the same face thetas), so one
+    //      more blend with the stage-2 face theta finishes them.
     double F2xp[7], F2yn[7];
     {
-      double tx[3], ty[3];
-      const double* tt[3] = {t0 + 1, t1 + 1, t2 + 1};  // rows f-3 at [0], f-2 at [1]
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        tx[k] = face_theta(tt[k][0], tt[k][1]);
-        ty[k] = face_theta(tt[k][1], lane_down(tt[k][1]));
-      }
+      const double tx = face_theta(t2[1], t2[2]);             // rows f-3, f-2
+      const double ty = face_theta(t2[2], lane_down(t2[2]));  // row f-2 and the lane after
 #pragma unroll
       for (int m = 0; m < 7; ++m) {
-        F2xp[m] = blend3(xa[0][m], Lx[0][m], tx, 3);
-        F2yn[m] = blend3(yn[m], yn[7 + m], ty, 3);
+        F2xp[m] = blend3(hm[m], Lx[0][m], &tx, 1);
+        F2yn[m] = blend3(hu[m], yn[7 + m], &ty, 1);
       }
     }
 
