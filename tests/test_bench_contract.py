@@ -33,12 +33,16 @@ def test_limiter_rates_from_the_committed_record():
     sys.path.insert(0, ROOT)
     import bench
 
-    r = bench.limiter_rates("mach10_cylinder_4096x2048", 0.861, 6000.0)
-    assert r["alg_bytes"] == 368 * 4096 * 2048
+    # the FP64 limiter at this size reads 14 face planes (112 B), U^k and U^n
+    # (96 B) and writes U^(k+1) (48 B): 256 B per cell
+    r = bench.limiter_rates("mach10_cylinder_4096x2048", "k_lu_lo", 0.861, 6000.0)
+    assert r["kernel"] == "k_lu_lo"
+    assert r["alg_bytes"] == 256 * 4096 * 2048
     assert r["traffic"] > r["alg_bytes"]
     assert abs(r["traffic_gbs"] - r["traffic"] / 0.861e-3 / 1e9) < 0.1
     assert abs(r["traffic_frac_of_copy"] - r["traffic_gbs"] / 6000.0) < 1e-4
-    assert bench.limiter_rates("mach10_cylinder_512x256", 0.1, 6000.0)["traffic"] is None
+    assert bench.limiter_rates("mach10_cylinder_512x256", "k_lu_lo", 0.1, 6000.0)["traffic"] is None
+    assert bench.limiter_rates("mach10_cylinder_4096x2048", "k_lu_march", 0.8, 6000.0)["traffic"] is None
 
 
 def test_cpu_device_leg_runs_on_the_host():
