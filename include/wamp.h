@@ -312,6 +312,23 @@ int wamp3_get_dt(wamp3_ctx *ctx, double *const U[7], double *dt_dev, void *strea
 /* One SSP-RK3 step in place on U, as wamp_step. */
 int wamp3_step(wamp3_ctx *ctx, double *const U[7], const double *dt_dev, double *dt_next_dev,
                void *stream);
+/* The split form of one stage, as wamp_compute_fluxes / wamp_limit_update
+   (the reference's compute_fluxes + compute_low_fluxes and limit_fluxes +
+   rk3_update + enforce_admissible + update_prims, SC:1919-1984, with the
+   z-sweep added).  F: 48 FP64 face planes of the padded grid, equally spaced
+   in one allocation; direction d (x, y, z) at planes d*16 .. d*16+15: the
+   high-order fluxes of the 7 conserved variables and u*, then the
+   first-order ones; the face d + 1/2 of a cell is stored at the cell's
+   index.  In mixed precision the face kernels' FP32 values are stored
+   widened, so the limiter sees the numbers wamp3_step does. */
+int wamp3_compute_fluxes(wamp3_ctx *ctx, double *const U[7], double *const F[48], void *stream);
+/* Uout = alpha * Un + beta * (U - dt R(U)) with the three Wong limiter
+   stages on F, repair and the a1 clamp; Uout must not alias U.  dt is read
+   from dt_dev; when dt_acc is non-NULL the CFL minimum of Uout is folded
+   into *dt_acc (an FP64 accumulator the caller initialises to 1e30). */
+int wamp3_limit_update(wamp3_ctx *ctx, double *const Un[7], double *const U[7],
+                       double *const F[48], double *const Uout[7], double alpha, double beta,
+                       const double *dt_dev, double *dt_acc, void *stream);
 
 #ifdef __cplusplus
 }

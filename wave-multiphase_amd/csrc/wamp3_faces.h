@@ -376,6 +376,9 @@ __device__ __forceinline__ void fstore3(const Faces3& F, int m, size_t c, double
       else F.h[(long long)face3_slot32(m) * F.hs + (long long)c] = (float)v;
       return;
     }
+    // FP64 storage of the mixed kernels' planes (the split entry points):
+    // the FP32 planes' values widened, as the limiter reads them.
+    if (!face3_is_energy(m)) v = (double)(float)v;
   }
   F.b[(long long)m * F.s + (long long)c] = v;
 }

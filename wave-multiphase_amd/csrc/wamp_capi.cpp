@@ -1073,6 +1073,57 @@ int wamp3_get_dt(wamp3_ctx* c, double* const U[7], double* dt_dev, void* stream)
   return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_dev, c->scheme.cfl, st), "3D dt");
 }
 
+static int faces3_of(const wamp3_ctx* c, double* const F[48], Faces3& f) {
+  if (!F) return fail(-2, "F is NULL");
+  for (int m = 0; m < 48; ++m)
+    if (!F[m]) return fail(-2, "F[%d] is NULL", m);
+  const long long st = (long long)(F[1] - F[0]);
+  const long long plane = (long long)(c->geo.nx + 2 * c->geo.g) * c->geo.sx;
+  if (st < plane && -st < plane)
+    return fail(-2, "face planes overlap (spacing %lld < plane %lld elements)", st, plane);
+  for (int m = 2; m < 48; ++m)
+    if ((long long)(F[m] - F[0]) != (long long)m * st)
+      return fail(-2, "the 48 face planes must be equally spaced in one allocation (F[%d])", m);
+  f = Faces3{F[0], st};
+  return 0;
+}
+
+int wamp3_compute_fluxes(wamp3_ctx* c, double* const U[7], double* const F[48], void* stream) {
+  if (!c) return fail(-1, "NULL context");
+  State3 s;
+  if (int rc = state3_of(U, s, "U")) return rc;
+  Faces3 f;
+  if (int rc = faces3_of(c, F, f)) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const wamp_scheme& sc = c->scheme;
+  if (int rc = hip_check(wamp3_launch_cells(&c->geo, &c->eos, &s, c->P, c->S, c->D, sc.shear, st),
+                         "3D cells"))
+    return rc;
+  auto faces = c->precision == 0 ? wamp3_launch_faces : wamp3_launch_faces_mixed;
+  return hip_check(faces(&c->geo, &c->eos, &s, c->P, c->S, c->D, &f, sc.base, sc.shear,
+                         sc.vf_thinc, sc.face_pp, c->t1, st),
+                   "3D faces");
+}
+
+int wamp3_limit_update(wamp3_ctx* c, double* const Un[7], double* const U[7], double* const F[48],
+                       double* const Uout[7], double alpha, double beta, const double* dt_dev,
+                       double* dt_acc, void* stream) {
+  if (!c) return fail(-1, "NULL context");
+  State3 sn, s, so;
+  if (int rc = state3_of(Un, sn, "Un")) return rc;
+  if (int rc = state3_of(U, s, "U")) return rc;
+  if (int rc = state3_of(Uout, so, "Uout")) return rc;
+  for (int m = 0; m < 7; ++m)
+    if (so.f[m] == s.f[m]) return fail(-2, "Uout[%d] aliases U[%d]", m, m);
+  if (!dt_dev) return fail(-2, "dt_dev must be non-NULL");
+  Faces3 f;
+  if (int rc = faces3_of(c, F, f)) return rc;
+  return hip_check(wamp3_launch_limit_update(&c->geo, &c->eos, &sn, &s, &f, c->th, &so, alpha,
+                                             beta, dt_dev, dt_acc, c->scheme.repair,
+                                             (hipStream_t)stream),
+                   "3D limit/update");
+}
+
 int wamp3_step(wamp3_ctx* c, double* const U[7], const double* dt_dev, double* dt_next_dev,
                void* stream) {
   if (!c) return fail(-1, "NULL context");

@@ -55,6 +55,7 @@ PtrArr7 = C.c_void_p * 7
 PtrArr6 = C.c_void_p * 6
 PtrArr3 = C.c_void_p * 3
 PtrArr28 = C.c_void_p * 28
+PtrArr48 = C.c_void_p * 48
 
 # (name, restype, argtypes) of every symbol include/wamp.h declares
 _VP = C.c_void_p
@@ -114,6 +115,9 @@ SIGNATURES = [
     ("wamp3_prepare", C.c_int, [_VP, PtrArr7, _VP]),
     ("wamp3_get_dt", C.c_int, [_VP, PtrArr7, _VP, _VP]),
     ("wamp3_step", C.c_int, [_VP, PtrArr7, _VP, _VP, _VP]),
+    ("wamp3_compute_fluxes", C.c_int, [_VP, PtrArr7, PtrArr48, _VP]),
+    ("wamp3_limit_update", C.c_int, [_VP, PtrArr7, PtrArr7, PtrArr48, PtrArr7, C.c_double,
+                                     C.c_double, _VP, _VP, _VP]),
 ]
 
 _lib = None
@@ -155,3 +159,7 @@ def ptrs3(planes) -> PtrArr3:
 
 def ptrs28(planes) -> PtrArr28:
     return PtrArr28(*[int(p) for p in planes])
+
+
+def ptrs48(planes) -> PtrArr48:
+    return PtrArr48(*[int(p) for p in planes])

@@ -314,6 +314,17 @@ class Solver3:
                                      self.stream), "wamp3_step")
         self.nstep += 1
 
+    def fluxes(self):
+        """The 48 face arrays of one stage's flux pass on the current state
+        (x, y, z: 8 high-order fluxes with u*, then the 8 first-order ones)."""
+        torch = _torch()
+        g = self.cfg.grid
+        F = torch.zeros((48, g.nxg, g.nyg, self.ldz), dtype=torch.float64, device=self.device)
+        _lib.check(self.L.wamp3_compute_fluxes(self.ctx, self._u,
+                                               _lib.ptrs48([F[m].data_ptr() for m in range(48)]),
+                                               self.stream), "wamp3_compute_fluxes")
+        return F[..., : g.nzg].cpu().numpy().copy()
+
     def run(self, nsteps: int):
         """n steps with dt from the state before each step; returns the dts."""
         torch = _torch()
