@@ -231,7 +231,7 @@ int wamp_batch_set_case(wamp_batch *b, int32_t slot, const wamp_grid *grid, cons
 int wamp_batch_clear_case(wamp_batch *b, int32_t slot);
 /* Work decomposition: tile 0 = 8x32 face tiles, 1 = 16x16; persistent blocks
    per CU for the face kernel (1..16) and the limiter kernel (0 = one wave per
-   limiter item, 1..16 persistent); limiter rows per wave (8/16/20/26/32).
+   limiter item, 1..16 persistent); limiter rows per wave (4..4096).
    Results do not depend on these. */
 int wamp_batch_set_tuning(wamp_batch *b, int32_t tile, int32_t faces_blocks_per_cu,
                           int32_t lu_blocks_per_cu, int32_t lu_seg);

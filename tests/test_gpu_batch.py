@@ -42,7 +42,8 @@ def test_batch_equals_alone_under_all_tunings(gpu_ok, precision):
     # limiter blocks per CU 0: one wave per item; 1, 2, 4: persistent waves
     # striding over the items
     for tuning, graph in ((Tuning(0, 3, 2, 16), 0), (Tuning(1, 2, 4, 8), 3), (Tuning(0, 1, 1, 32), 6),
-                          (Tuning(1, 3, 2, 26), 2), (Tuning(0, 3, 0, 20), 0), (Tuning(1, 3, 0, 26), 2)):
+                          (Tuning(1, 3, 2, 26), 2), (Tuning(0, 3, 0, 20), 0), (Tuning(1, 3, 0, 26), 2),
+                          (Tuning(0, 3, 0, 5), 0), (Tuning(1, 2, 2, 100), 2)):
         b = Batch(cs[0][0].scheme, precision, max_cases=6)
         for k, (c, U) in enumerate(cs):
             b.set_case(k + 1, c, U)  # slot 0 and 5 stay empty

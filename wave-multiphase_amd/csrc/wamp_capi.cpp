@@ -320,57 +320,54 @@ static int faces_of(const wamp_ctx* c, double* const F[28], Faces& f) {
   return 0;
 }
 
-// Rows per wave of the marching limiter.  A wave of `seg` rows marches
-// seg + 3 fronts (the 3-row warm-up re-reads rows of the segment below), and
-// the kernel's blocks (4 waves of one strip) run in rounds of 2 blocks per
-// CU.  Cost model: rounds x (seg + 3), over the instantiated lengths; it
-// picks the measured best at 1024x512 (8 rows: 0.098 ms per launch; 16:
-// 0.111), 2048x1024 (20: 0.231; 16: 0.311) and 4096x2048 (26: 0.861; 16:
-// 0.916).
+// Rows per wave of the marching limiter (k_lu_march).  Its waves, one per
+// (58-column strip, segment), are packed 4 to a block and run two blocks per
+// CU, so the device holds 8 x CUs of them at a time; a wave of `seg` rows
+// runs seg + 6 fronts.  Cost: rounds x (seg + 6), rounds = waves / (8 x CUs)
+// rounded up; for each round count the shortest segment whose waves fit is
+// the candidate, as for k_lu_lo below.
 constexpr int MJ_COLS = 58;  // output columns per limiter wave (64 lanes - 2 x 3 halo)
-static int lu_rows(const StepPlan* c) {
-  if (c->lu_seg) return c->lu_seg;
-  static const int cand[] = {4, 8, 16, 20, 26, 28, 32};
-  const long long strips = (c->geo.ny + MJ_COLS - 1) / MJ_COLS;
-  const long long slots = 2LL * std::max(1, c->cu_count);
-  int best = 16;
-  long long best_cost = -1;
-  for (int seg : cand) {
-    const long long segs = (c->geo.nx + seg - 1) / seg;
-    const long long blocks = strips * ((segs + 3) / 4);
-    const long long cost = ((blocks + slots - 1) / slots) * (seg + 3);
+static int rows_for_rounds(long long nx, long long strips, long long slots, int fronts_extra,
+                           bool paired) {
+  const long long max_seg = std::max(4LL, nx);
+  long long best = max_seg, best_cost = -1;
+  for (long long r = 1;; ++r) {
+    const long long nseg = std::max(1LL, r * slots / strips);  // segments per strip in r rounds
+    const long long seg = std::max(4LL, std::min(max_seg, (nx + nseg - 1) / nseg));
+    const long long waves = strips * ((nx + seg - 1) / seg);
+    const long long rounds = (waves + slots - 1) / slots;
+    long long fronts = seg + fronts_extra;
+    if (paired) fronts = (fronts + 1) & ~1LL;
+    const long long cost = rounds * fronts;
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
       best = seg;
     }
+    if (seg == 4 || r > 64) break;
   }
-  return best;
+  return (int)best;
+}
+static int lu_rows(const StepPlan* c) {
+  if (c->lu_seg) return c->lu_seg;
+  const long long strips = (c->geo.ny + MJ_COLS - 1) / MJ_COLS;
+  return rows_for_rounds(c->geo.nx, strips, 8LL * std::max(1, c->cu_count), 6, false);
 }
 
 // Rows per wave of the march that forms the first-order fluxes itself
-// (k_lu_lo): its blocks (4 waves of one strip) run one per CU and a wave of
-// `seg` rows runs seg + 6 fronts (seg + 7 when that is odd: the fronts run in
-// pairs), so the cost is rounds x (seg + 6) with one block slot per CU and 56
-// output columns per wave.  At 4096x2048 it picks 32 rows, the measured best
-// (ms per launch: 16 rows 1.118, 26 0.992, 32 0.966, 48 1.089, 64 1.027).
+// (k_lu_lo).  Its waves, one per (56-column strip, segment), are packed 4 to
+// a block and run one block per CU, so the device holds 4 x CUs of them at a
+// time; a wave of `seg` rows runs seg + 6 fronts (seg + 7 when that is odd:
+// the fronts run in pairs).  Cost: rounds x fronts, where rounds = waves /
+// (4 x CUs) rounded up.  For each round count R the shortest segment whose
+// waves fit in R rounds is the candidate, and the cheapest candidate wins:
+// at 4096x2048 on 256 CUs one round of 27 segments of 152 rows (158 fronts)
+// instead of five rounds of 32-row segments (190 fronts).  Segments stay at
+// 4 rows or more.
 constexpr int LO_COLS = 56;
 static int lu_rows_lo(const StepPlan* c) {
   if (c->lu_seg) return c->lu_seg;
-  static const int cand[] = {8, 16, 26, 32, 48, 64};
   const long long strips = (c->geo.ny + LO_COLS - 1) / LO_COLS;
-  const long long slots = std::max(1, c->cu_count);
-  int best = 16;
-  long long best_cost = -1;
-  for (int seg : cand) {
-    const long long segs = (c->geo.nx + seg - 1) / seg;
-    const long long blocks = strips * ((segs + 3) / 4);
-    const long long cost = ((blocks + slots - 1) / slots) * (seg + 6);
-    if (best_cost < 0 || cost < best_cost) {
-      best_cost = cost;
-      best = seg;
-    }
-  }
-  return best;
+  return rows_for_rounds(c->geo.nx, strips, 4LL * std::max(1, c->cu_count), 6, true);
 }
 
 // Limiter kernel: the LDS-tile kernel (every tile in parallel, four short
@@ -783,8 +780,7 @@ int wamp_batch_set_tuning(wamp_batch* b, int32_t tile, int32_t faces_bpcu, int32
   if (tile < 0 || tile > 1) return fail(-3, "tile must be 0 (8x32) or 1 (16x16)");
   if (faces_bpcu < 1 || faces_bpcu > 16) return fail(-3, "face blocks per CU must be in [1, 16]");
   if (lu_bpcu < 0 || lu_bpcu > 16) return fail(-3, "limiter blocks per CU must be in [0, 16]");
-  if (lu_seg != 8 && lu_seg != 16 && lu_seg != 20 && lu_seg != 26 && lu_seg != 32)
-    return fail(-3, "lu_seg must be 8, 16, 20, 26 or 32");
+  if (lu_seg < 4 || lu_seg > 4096) return fail(-3, "lu_seg must be in [4, 4096]");
   b->tile = tile;
   b->faces_bpcu = faces_bpcu;
   b->lu_bpcu = lu_bpcu;

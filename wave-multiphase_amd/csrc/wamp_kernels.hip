@@ -187,19 +187,19 @@ __device__ __forceinline__ double blend3(double H, double L, const double* th, i
   return f;
 }
 
-// One wave's work item: columns strip*58-3 .. +64, rows seg*SEG .. +SEG.
+// One wave's work item: columns strip*58-3 .. +64, rows seg*rows .. +rows.
 // `cw` is this wave's LU_CARRY x 64 doubles of LDS.  Returns this lane's CFL
 // minimum of the updated cells (1e30 if none or if want_dt is false).
-template <int SEG, bool F32>
+template <bool F32>
 __device__ __forceinline__ double lu_wave(const Geo& G, const Eos& e, const State& Un,
                                           const State& U, const Faces& F, const State& Uo,
                                           double alpha, double beta, double dt, int repair,
-                                          bool want_dt, int strip, int seg, double* cw) {
+                                          bool want_dt, int strip, int seg, int rows, double* cw) {
   const int lane = threadIdx.x & (MW - 1);
   const int j = G.g + strip * MJ - MH + lane;
-  const int i0 = G.g + seg * SEG;
+  const int i0 = G.g + seg * rows;
   if (i0 >= G.g + G.nx) return 1.0e30;  // wave-uniform
-  const int i_end = min(i0 + SEG, G.g + G.nx);
+  const int i_end = min(i0 + rows, G.g + G.nx);
   const size_t ld = (size_t)G.ld;
   const double inv_dx = 1.0 / G.dx;
   const double inv_dy = 1.0 / G.dy;
@@ -560,27 +560,37 @@ __device__ __forceinline__ void wave_min_atomic(double v, double* acc) {
     atomicMin((unsigned long long*)acc, (unsigned long long)__double_as_longlong(v));
 }
 
-template <int SEG, bool F32>
+// One wave per (strip, segment of `rows` rows), numbered strip-fastest and
+// packed 4 to a block, so a strip's segment count need not be a multiple of
+// 4: the host picks `rows` so that the waves fill the device's two-blocks-
+// per-CU slots in as few rounds as it can.  (Strip-fastest measured faster
+// than segment-fastest for this kernel, two waves per SIMD, whose loads wait
+// on memory: at 4096x2048 on the mixed storage 0.518 against 0.565 ms per
+// launch at 74 rows, 0.559 against 0.698 ms at 26 rows.)
+template <bool F32>
 __global__ void __launch_bounds__(256, 2)
 k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
-           const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+           const double* __restrict__ dt_dev, double* dt_acc, int repair, int rows, int nseg,
+           int nitems) {
   __shared__ double carry[LU_WAVES * LU_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
   // Workgroups go to the 8 XCDs round-robin by linear id.  Give each XCD a
-  // contiguous run of (strip, segment group) in strip-fastest order, so that
-  // neighbouring strips, which share their halo columns' cache lines, and
-  // neighbouring segment groups, which share warm-up rows, run on the same
-  // XCD's L2.  The map is a bijection on [0, N); which block does which strip
-  // and segment does not change any result.
-  const int nblk = (int)(gridDim.x * gridDim.y);
-  const int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  // contiguous run of blocks, i.e. of whole rows of neighbouring strips, so
+  // that neighbouring strips, which share their halo columns' cache lines,
+  // and neighbouring segments, which share warm-up rows, run on the same
+  // XCD's L2.  The map is a bijection on [0, N); which block does which
+  // strip and segment does not change any result.
+  const int nblk = (int)gridDim.x;
+  const int lin = (int)blockIdx.x;
   const int xcd = lin & 7, base = nblk >> 3, extra = nblk & 7;
   const int q = xcd * base + (xcd < extra ? xcd : extra) + (lin >> 3);
-  const int strip = q % (int)gridDim.x;
-  const int seg = (q / (int)gridDim.x) * (blockDim.x / MW) + wave;
-  if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
-  double v = lu_wave<SEG, F32>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
-                          strip, seg, carry + wave * (LU_CARRY * MW));
+  const int item = q * LU_WAVES + wave;
+  if (item >= nitems) return;  // wave-uniform
+  const int nstrip = nitems / nseg;  // nitems = strips x nseg
+  const int seg = item / nstrip;
+  const int strip = item - seg * nstrip;
+  double v = lu_wave<F32>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
+                          strip, seg, rows, carry + wave * (LU_CARRY * MW));
   if (dt_acc) wave_min_atomic(v, dt_acc);
 }
 
@@ -592,7 +602,8 @@ k_lu_march(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, dou
 // march streams those cells' state anyway.  Per launch that removes 112 B per
 // cell of face writes and 112 B per cell of face reads, and the first-order
 // HLLC moves out of the face kernel, which is bound by VALU issue, into this
-// one, which is bound by memory.
+// one, which runs at one wave per SIMD and is bound by that wave's
+// instruction issue.
 //
 // One wave owns 64 consecutive j: 56 output columns and a 4-lane halo each
 // side.  The y-face above lane l is formed from the cells of lanes l and l+1
@@ -686,16 +697,16 @@ __device__ __forceinline__ void lo_cell(const Eos& e, const double* q, double sg
   E = prim_energy(e, P[0], P[1], P[2], P[3], P[4], P[5]);
 }
 
-template <int SEG>
 __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const State& Un,
                                              const State& U, const Faces& F, const State& Uo,
                                              double alpha, double beta, double dt, int repair,
-                                             bool want_dt, int strip, int seg, double* cw) {
+                                             bool want_dt, int strip, int seg, int rows,
+                                             double* cw) {
   const int lane = threadIdx.x & (MW - 1);
   const int j = G.g + strip * LOJ - LOH + lane;  // >= g - 4 >= 0
-  const int i0 = G.g + seg * SEG;
+  const int i0 = G.g + seg * rows;
   if (i0 >= G.g + G.nx) return 1.0e30;  // wave-uniform
-  const int i_end = min(i0 + SEG, G.g + G.nx);
+  const int i_end = min(i0 + rows, G.g + G.nx);
   const double inv_dx = 1.0 / G.dx;
   const double inv_dy = 1.0 / G.dy;
   const int jc = min(j, G.ld - 1);
@@ -1016,22 +1027,30 @@ __device__ __forceinline__ double lu_wave_lo(const Geo& G, const Eos& e, const S
   return local_dt;
 }
 
-template <int SEG>
+// One wave per (strip, segment of `rows` rows), numbered segment-fastest and
+// packed LO_WAVES to a block, so a strip's segment count need not be a
+// multiple of LO_WAVES: the host picks `rows` so that the waves fill the
+// device's one-block-per-CU slots in as few rounds as it can.  (Segment-
+// fastest measured faster than strip-fastest for this kernel: at 4096x2048,
+// 152 rows, 0.692 against 0.722 ms per launch.)
 __global__ void __launch_bounds__(64 * LO_WAVES, 1)
 k_lu_lo(Geo G, Eos e, State Un, State U, Faces F, State Uo, double alpha, double beta,
-        const double* __restrict__ dt_dev, double* dt_acc, int repair) {
+        const double* __restrict__ dt_dev, double* dt_acc, int repair, int rows, int nseg,
+        int nitems) {
   __shared__ double carry[LO_WAVES * LO_CARRY * MW];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / MW);  // wave-uniform
-  // the XCD-aware block map of k_lu_march (a bijection on [0, N))
-  const int nblk = (int)(gridDim.x * gridDim.y);
-  const int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  // the XCD-aware block map of k_lu_march (a bijection on [0, N)): each XCD
+  // takes a contiguous run of blocks, i.e. of whole neighbouring strips
+  const int nblk = (int)gridDim.x;
+  const int lin = (int)blockIdx.x;
   const int xcd = lin & 7, base = nblk >> 3, extra = nblk & 7;
   const int q = xcd * base + (xcd < extra ? xcd : extra) + (lin >> 3);
-  const int strip = q % (int)gridDim.x;
-  const int seg = (q / (int)gridDim.x) * LO_WAVES + wave;
-  if (G.g + seg * SEG >= G.g + G.nx) return;  // wave-uniform
-  double v = lu_wave_lo<SEG>(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr,
-                             strip, seg, carry + wave * (LO_CARRY * MW));
+  const int item = q * LO_WAVES + wave;
+  if (item >= nitems) return;  // wave-uniform
+  const int strip = item / nseg;
+  const int seg = item - strip * nseg;
+  double v = lu_wave_lo(G, e, Un, U, F, Uo, alpha, beta, *dt_dev, repair, dt_acc != nullptr, strip,
+                        seg, rows, carry + wave * (LO_CARRY * MW));
   if (dt_acc) wave_min_atomic(v, dt_acc);
 }
 
@@ -1186,11 +1205,12 @@ __global__ void __launch_bounds__(STHREADS) k_bc(Geo G, State U, double* up, dou
 // device, so a captured step graph needs no host interaction.
 // ========================================================================
 
-// One wave per work item (case, strip, segment) of the concatenated list.
-template <int SEG, bool F32>
+// One wave per work item (case, strip, segment of `rows` rows) of the
+// concatenated list.
+template <bool F32>
 __global__ void __launch_bounds__(256, 2)
 k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, int ncase,
-           BatchScal sc, int stage, int repair) {
+           BatchScal sc, int stage, int repair, int rows) {
   // the wave index is uniform across the wave: keep it (and everything derived
   // from it: case, descriptor, rows) in scalar registers
   __shared__ double carry[LU_WAVES * LU_CARRY * MW];
@@ -1213,9 +1233,9 @@ k_lu_batch(const CaseDesc* __restrict__ cs, const int* __restrict__ item_off, in
       U.f[m] = c.u[stage][m];
       Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
     }
-    double v = lu_wave<SEG, F32>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair,
-                                 stage == 2, li % c.strips, li / c.strips,
-                                 carry + wave * (LU_CARRY * MW));
+    double v = lu_wave<F32>(c.geo, c.eos, Un, U, c.f, Uo, alpha, beta, sc.dt[k], repair,
+                            stage == 2, li % c.strips, li / c.strips, rows,
+                            carry + wave * (LU_CARRY * MW));
     if (stage == 2) wave_min_atomic(v, sc.acc + k);
   }
 }
@@ -1394,42 +1414,35 @@ hipError_t wamp_launch_lu_march(const Geo* G, const Eos* e, const State* Un, con
                                const Faces* F, const State* Uo, double alpha, double beta,
                                const double* dt_dev, double* dt_acc, int repair, int seg,
                                hipStream_t st) {
-  const int waves_per_block = 4;
-  const int nseg = (G->nx + seg - 1) / seg;
-  dim3 grid((G->ny + MJ - 1) / MJ, (nseg + waves_per_block - 1) / waves_per_block);
-#define WAMP_SEG(S)                                                                             \
-  if (seg == S) {                                                                               \
-    if (F->h)                                                                                   \
-      hipLaunchKernelGGL((k_lu_march<S, true>), grid, dim3(64 * waves_per_block), 0, st, *G, *e, \
-                         *Un, *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);                \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_lu_march<S, false>), grid, dim3(64 * waves_per_block), 0, st, *G,   \
-                         *e, *Un, *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair);            \
-    return hipGetLastError();                                                                   \
-  }
-  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(20) WAMP_SEG(26) WAMP_SEG(28) WAMP_SEG(32) WAMP_SEG(64)
-#undef WAMP_SEG
-  return hipErrorInvalidValue;
+  if (seg < 1) return hipErrorInvalidValue;
+  const long long nseg = (G->nx + seg - 1) / seg;
+  const long long items = (long long)((G->ny + MJ - 1) / MJ) * nseg;
+  if (items > (1LL << 30)) return hipErrorInvalidValue;
+  const int blocks = (int)((items + LU_WAVES - 1) / LU_WAVES);
+  if (F->h)
+    hipLaunchKernelGGL(k_lu_march<true>, dim3(blocks), dim3(64 * LU_WAVES), 0, st, *G, *e, *Un, *U,
+                       *F, *Uo, alpha, beta, dt_dev, dt_acc, repair, seg, (int)nseg, (int)items);
+  else
+    hipLaunchKernelGGL(k_lu_march<false>, dim3(blocks), dim3(64 * LU_WAVES), 0, st, *G, *e, *Un,
+                       *U, *F, *Uo, alpha, beta, dt_dev, dt_acc, repair, seg, (int)nseg,
+                       (int)items);
+  return hipGetLastError();
 }
 
 // The march that forms the first-order fluxes itself (FP64 face storage; the
-// first-order planes of F are not read).
+// first-order planes of F are not read), `seg` rows per wave (any seg >= 1).
 hipError_t wamp_launch_lu_lo(const Geo* G, const Eos* e, const State* Un, const State* U,
                              const Faces* F, const State* Uo, double alpha, double beta,
                              const double* dt_dev, double* dt_acc, int repair, int seg,
                              hipStream_t st) {
-  if (F->h) return hipErrorInvalidValue;
-  const int nseg = (G->nx + seg - 1) / seg;
-  dim3 grid((G->ny + LOJ - 1) / LOJ, (nseg + LO_WAVES - 1) / LO_WAVES);
-#define WAMP_SEG(S)                                                                           \
-  if (seg == S) {                                                                             \
-    hipLaunchKernelGGL((k_lu_lo<S>), grid, dim3(64 * LO_WAVES), 0, st, *G, *e, *Un, *U, *F,    \
-                       *Uo, alpha, beta, dt_dev, dt_acc, repair);                             \
-    return hipGetLastError();                                                                 \
-  }
-  WAMP_SEG(4) WAMP_SEG(8) WAMP_SEG(16) WAMP_SEG(26) WAMP_SEG(32) WAMP_SEG(48) WAMP_SEG(64)
-#undef WAMP_SEG
-  return hipErrorInvalidValue;
+  if (F->h || seg < 1) return hipErrorInvalidValue;
+  const long long nseg = (G->nx + seg - 1) / seg;
+  const long long items = (long long)((G->ny + LOJ - 1) / LOJ) * nseg;
+  if (items > (1LL << 30)) return hipErrorInvalidValue;
+  const int blocks = (int)((items + LO_WAVES - 1) / LO_WAVES);
+  hipLaunchKernelGGL(k_lu_lo, dim3(blocks), dim3(64 * LO_WAVES), 0, st, *G, *e, *Un, *U, *F, *Uo,
+                     alpha, beta, dt_dev, dt_acc, repair, seg, (int)nseg, (int)items);
+  return hipGetLastError();
 }
 
 hipError_t wamp_launch_dt(const Geo* G, const Eos* e, const State* U, double* dt_acc,
@@ -1538,19 +1551,14 @@ hipError_t wamp_launch_faces_batch(const CaseDesc* cs, const int* tile_off, int 
 hipError_t wamp_launch_lu_batch(const CaseDesc* cs, const int* item_off, int ncase,
                                 const BatchScal* sc, int stage, int repair, int seg, int blocks,
                                 int f32, hipStream_t st) {
-#define WAMP_LB(S)                                                                              \
-  if (seg == S) {                                                                               \
-    if (f32)                                                                                    \
-      hipLaunchKernelGGL((k_lu_batch<S, true>), dim3(blocks), dim3(256), 0, st, cs, item_off,    \
-                         ncase, *sc, stage, repair);                                            \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_lu_batch<S, false>), dim3(blocks), dim3(256), 0, st, cs, item_off,   \
-                         ncase, *sc, stage, repair);                                            \
-    return hipGetLastError();                                                                   \
-  }
-  WAMP_LB(8) WAMP_LB(16) WAMP_LB(20) WAMP_LB(26) WAMP_LB(32)
-#undef WAMP_LB
-  return hipErrorInvalidValue;
+  if (seg < 1) return hipErrorInvalidValue;
+  if (f32)
+    hipLaunchKernelGGL(k_lu_batch<true>, dim3(blocks), dim3(256), 0, st, cs, item_off, ncase, *sc,
+                       stage, repair, seg);
+  else
+    hipLaunchKernelGGL(k_lu_batch<false>, dim3(blocks), dim3(256), 0, st, cs, item_off, ncase,
+                       *sc, stage, repair, seg);
+  return hipGetLastError();
 }
 
 hipError_t wamp_launch_batch_finalize(const CaseDesc* cs, int ncase, const BatchScal* sc,

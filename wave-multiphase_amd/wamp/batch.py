@@ -35,7 +35,7 @@ class Tuning:
         return (self.tile, self.faces_bpcu, self.lu_bpcu, self.lu_seg)
 
 
-TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (0, 2) for s in (8, 16, 26)]
+TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (0, 2) for s in (16, 26, 48)]
 
 
 class Batch:
