@@ -174,8 +174,8 @@ __device__ __forceinline__ void face_state_pp3(const Eos& e, double* hi, const d
 #pragma unroll
   for (int m = 0; m < 7; ++m) b[m] = base[m];
   b[6] = clamp01_pp(b[6]);
-  if (b[0] < PP_RHO || isnan_(b[0])) b[0] = PP_RHO;
-  if (b[1] < PP_RHO || isnan_(b[1])) b[1] = PP_RHO;
+  b[0] = floor_pp(b[0], PP_RHO);
+  b[1] = floor_pp(b[1], PP_RHO);
   wong_lower3(b, hi, b[0], hi[0], PP_RHO);
   wong_lower3(b, hi, b[1], hi[1], PP_RHO);
   wong_lower3(b, hi, b[6], hi[6], PP_ALPHA);

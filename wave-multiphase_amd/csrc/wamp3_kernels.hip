@@ -36,8 +36,8 @@ namespace wamp {
 __device__ __forceinline__ double rho_ct2_cons3(const Eos& e, const double* q) {
   double a1 = clamp01_pp(q[6]);
   double m1 = q[0], m2 = q[1];
-  if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
-  if (m2 < PP_RHO || isnan_(m2)) m2 = PP_RHO;
+  m1 = floor_pp(m1, PP_RHO);
+  m2 = floor_pp(m2, PP_RHO);
   double rho = m1 + m2;
   double uu = ddiv(q[2], rho);
   double vv = ddiv(q[3], rho);
@@ -234,8 +234,8 @@ __global__ void __launch_bounds__(256) k3_theta(Geo3 G, Eos e, State3 U, Faces3 
 __device__ __forceinline__ void repair_cell3(const Eos& e, double* u) {
   double a1 = clamp01_pp(u[6]);
   double m1 = u[0], m2 = u[1], mx = u[2], my = u[3], mz = u[4], E = u[5];
-  if (isnan_(m1) || m1 < PP_RHO) m1 = PP_RHO;
-  if (isnan_(m2) || m2 < PP_RHO) m2 = PP_RHO;
+  m1 = floor_pp(m1, PP_RHO);
+  m2 = floor_pp(m2, PP_RHO);
   if (isnan_(mx)) mx = 0.0;
   if (isnan_(my)) my = 0.0;
   if (isnan_(mz)) mz = 0.0;

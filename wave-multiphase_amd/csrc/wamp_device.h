@@ -274,15 +274,20 @@ WAMP_FN T prim_energy(const Eos& e, T ar1, T ar2, T u, T v, T p, T a1) {
   mix_gb_pb(e, a1, gb, pb);
   return gb * p + pb + T(0.5) * rho * (u * u + v * v);
 }
-// clamp01_pp SC:112-121
-template <typename T>
-WAMP_FN T clamp01_pp(T a1) {
-  T a = a1;
-  if (isnan_(a)) a = T(PP_ALPHA);
-  if (a < T(PP_ALPHA)) a = T(PP_ALPHA);
-  if (a > T(1) - T(PP_ALPHA)) a = T(1) - T(PP_ALPHA);
-  return a;
+// clamp01_pp SC:112-121: NaN -> PP_ALPHA, then clamped to [PP_ALPHA,
+// 1 - PP_ALPHA].  fmax returns its other operand for a NaN argument (IEEE
+// maxNum), so fmin(fmax(a, lo), hi) is the reference's branch sequence value
+// for value: one v_max and one v_min instead of compares, selects and
+// constant moves.
+WAMP_FN double clamp01_pp(double a) {
+  return __builtin_fmin(__builtin_fmax(a, PP_ALPHA), 1.0 - PP_ALPHA);
 }
+WAMP_FN float clamp01_pp(float a) {
+  return __builtin_fminf(__builtin_fmaxf(a, (float)PP_ALPHA), 1.0f - (float)PP_ALPHA);
+}
+// x < lo || isnan(x) ? lo : x (the partial-density floors SC:223-226,
+// SC:249-252 and SC:933-936), as one fmax for the same reason.
+WAMP_FN double floor_pp(double x, double lo) { return __builtin_fmax(x, lo); }
 // pressure recovery from the conserved state, cons_to_prim_kernel SC:856-860
 // (always FP64: p is the small difference of E and the p_inf term).
 WAMP_FN double pressure(const Eos& e, double ar1, double ar2, double ru,
@@ -333,8 +338,8 @@ WAMP_FN double theta_lower(double base, double high, double floor_) {
 WAMP_FN double rho_ct2_cons(const Eos& e, double m1, double m2, double mx,
                                                double my, double E, double a1_in) {
   double a1 = clamp01_pp(a1_in);
-  if (m1 < PP_RHO || isnan_(m1)) m1 = PP_RHO;
-  if (m2 < PP_RHO || isnan_(m2)) m2 = PP_RHO;
+  m1 = floor_pp(m1, PP_RHO);
+  m2 = floor_pp(m2, PP_RHO);
   double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
   double uu = dv(mx, rho);
   double vv = dv(my, rho);
@@ -565,9 +570,21 @@ WAMP_FN void to_phys(const Basis<T>& b, const T* w, T* s) {
 }
 
 // ----------------------------------------------- Wong face limiter ---------
+// clamp01_pp as the reference's branches (compares and selects).  The FP64
+// face kernels with the face positivity limiter run at the 168 VGPRs that
+// three waves per SIMD allow; with rho_ct2_sc's clamp in this form their
+// registers fit (the v_max / v_min form there made the M10 kernel spill 8
+// values to scratch and run 7 % slower).
+WAMP_FN double clamp01_sel(double a1) {
+  double a = a1;
+  if (isnan_(a)) a = PP_ALPHA;
+  if (a < PP_ALPHA) a = PP_ALPHA;
+  if (a > 1.0 - PP_ALPHA) a = 1.0 - PP_ALPHA;
+  return a;
+}
 // rho_ctilde2_sc SC:174-178
 WAMP_FN double rho_ct2_sc(const Eos& e, const double* s) {
-  double a1 = clamp01_pp(s[5]);
+  double a1 = clamp01_sel(s[5]);
   double g, pi;
   mix_g_pi(e, a1, g, pi);
   return ddiv(s[4] + pi, g - 1.0);
@@ -598,8 +615,8 @@ WAMP_FN void face_state_pp(const Eos& e, double* hi, const double* base) {
 #pragma unroll
   for (int m = 0; m < 6; ++m) b[m] = base[m];
   b[5] = clamp01_pp(b[5]);
-  if (b[0] < PP_RHO || isnan_(b[0])) b[0] = PP_RHO;
-  if (b[1] < PP_RHO || isnan_(b[1])) b[1] = PP_RHO;
+  b[0] = floor_pp(b[0], PP_RHO);
+  b[1] = floor_pp(b[1], PP_RHO);
   wong_lower(b, hi, b[0], hi[0], PP_RHO);
   wong_lower(b, hi, b[1], hi[1], PP_RHO);
   wong_lower(b, hi, b[5], hi[5], PP_ALPHA);

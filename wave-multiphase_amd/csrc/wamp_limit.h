@@ -64,8 +64,8 @@ WAMP_FN void residual(const Faces& F, int m, size_t c, size_t ld,
 WAMP_FN void repair_cell(const Eos& e, double* u) {
   double a1 = clamp01_pp(u[5]);
   double m1 = u[0], m2 = u[1], mx = u[2], my = u[3], E = u[4];
-  if (isnan_(m1) || m1 < PP_RHO) m1 = PP_RHO;
-  if (isnan_(m2) || m2 < PP_RHO) m2 = PP_RHO;
+  m1 = floor_pp(m1, PP_RHO);
+  m2 = floor_pp(m2, PP_RHO);
   if (isnan_(mx)) mx = 0.0;
   if (isnan_(my)) my = 0.0;
   double rho = m1 + m2;  // >= 2e-10: both partial densities are floored
