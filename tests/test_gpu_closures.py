@@ -337,8 +337,12 @@ def test_limiter_kernels_agree_under_heavy_limiting(gpu_ok, monkeypatch):
 
     si, sj = g.interior()
     outs = {}
-    variants = ([("0", "0")] + [("1", s) for s in ("4", "8", "16", "20", "26", "28", "32", "64")]
-                + [("2", s) for s in ("4", "8", "16", "26", "32", "48", "64")])
+    # rows per wave are run-time values: also 1, odd counts, the context's own
+    # choice ("0") and one segment per strip (1000 rows > nx)
+    variants = ([("0", "0")] + [("1", s) for s in ("4", "8", "16", "20", "26", "28", "32", "64",
+                                                   "1", "5", "77", "0", "1000")]
+                + [("2", s) for s in ("4", "8", "16", "26", "32", "48", "64",
+                                      "1", "5", "151", "0", "1000")])
     for kind, seg in variants:
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)

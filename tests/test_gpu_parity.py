@@ -134,9 +134,9 @@ def test_limiter_kernels_agree_bitwise(gpu_ok, scheme, precision, monkeypatch):
     U = orc.state()
     dt = orc.get_dt()
     outs = {}
-    segs = ("4", "8", "16", "20", "26", "28", "32", "64")
+    segs = ("4", "8", "16", "20", "26", "28", "32", "64", "3", "37", "0")
     # the march that forms the first-order fluxes itself reads FP64 face storage only
-    segs_lo = ("4", "8", "16", "26", "32", "48", "64") if precision == 0 else ()
+    segs_lo = ("4", "8", "16", "26", "32", "48", "64", "3", "37", "0") if precision == 0 else ()
     for kind, seg in ((("0", "32"),) + tuple(("1", s) for s in segs)
                       + tuple(("2", s) for s in segs_lo)):
         monkeypatch.setenv("WAMP_LU_KIND", kind)
@@ -168,7 +168,8 @@ def test_march_limiter_on_degenerate_grids(gpu_ok, nx, ny, precision, monkeypatc
     cfg = dataclasses.replace(cfg, precision=precision)
     U = orc.state()
     outs = {}
-    for kind, seg in (("0", "0"), ("1", "4"), ("1", "26"), ("2", "4"), ("2", "26")):
+    for kind, seg in (("0", "0"), ("1", "4"), ("1", "26"), ("2", "4"), ("2", "26"), ("1", "0"),
+                      ("2", "0")):
         monkeypatch.setenv("WAMP_LU_KIND", kind)
         monkeypatch.setenv("WAMP_LU_SEG", seg)
         s = Solver(cfg)
