@@ -188,6 +188,76 @@ def threed_leg(n: int, warmup: int, steps: int):
                                                         if k != "finite"}))
 
 
+def threed_batched_leg(n_cases: int, steps: int, seed: int = 6):
+    """Uneven 3D shock-sphere cases (2n x n x n, n from {8, 12, 16, 24, 32},
+    Mach uniform in [1.5, 4], M10): Solver3 one case after another in FP64
+    against one 3D batch (each kernel of a stage one launch over every case,
+    a 10-step graph) in FP64 and in mixed precision.  Steps only are timed,
+    setup apart; the FP64 batch must equal the plain runs bit for bit."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from wamp import threed
+    from wamp.config import SCHEME_M10
+
+    rng = np.random.default_rng(seed)
+    work = [threed.shock_sphere(int(rng.choice([8, 12, 16, 24, 32])), SCHEME_M10,
+                                float(rng.uniform(1.5, 4.0))) for _ in range(n_cases)]
+    cells = sum(c.grid.nx * c.grid.ny * c.grid.nz for c, _ in work)
+    plain, t_plain = [], 0.0
+    for cfg, U0 in work:
+        s = threed.Solver3(cfg)
+        s.set_state(U0)
+        s.prepare()
+        s.compute_dt()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step_device()
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        plain.append(s.state())
+        s.close()
+    res, states = {}, {}
+    for prec in (0, 1):
+        b = threed.Batch3(SCHEME_M10, prec, max_cases=n_cases)
+        for k, (cfg, U0) in enumerate(work):
+            b.set_case(k, dataclasses.replace(cfg, precision=prec), U0)
+        b.commit(graph_steps=10 if steps % 10 == 0 else 1)
+        b.prepare()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            done += b.step()
+        torch.cuda.synchronize()
+        res[prec] = time.perf_counter() - t0
+        states[prec] = [b.state(k) for k in range(n_cases)]
+        b.close()
+    same = all(np.array_equal(a, p_) for a, p_ in zip(states[0], plain))
+    err = {}
+    for (cfg, _), a, p_ in zip(work, states[1], states[0]):
+        for key, v in threed.compare3(cfg, a, p_).items():
+            if key != "finite":
+                err[key] = max(err.get(key, 0.0), float(v))
+    cph = lambda t: round(n_cases / t * 3600.0, 1)
+    return {"workload": f"{n_cases} shock-sphere cases, 2n x n x n with n in {{8, 12, 16, 24, 32}}, "
+                        f"Mach U[1.5, 4], M10, seed {seed}",
+            "cells": cells, "steps_per_case": steps, "timing": "steps only",
+            "plain_fp64_cases_per_hour": cph(t_plain),
+            "batched_fp64_cases_per_hour": cph(res[0]),
+            "batched_mixed_cases_per_hour": cph(res[1]),
+            "batching_speedup_fp64": round(t_plain / res[0], 3),
+            "speedup": round(t_plain / res[1], 3),
+            "batched_fp64_gcell_updates_per_s": round(cells * steps / res[0] / 1e9, 4),
+            "batched_mixed_gcell_updates_per_s": round(cells * steps / res[1] / 1e9, 4),
+            "plain_fp64_gcell_updates_per_s": round(cells * steps / t_plain / 1e9, 4),
+            "batched_fp64_equals_plain_bitwise": same,
+            "mixed_vs_fp64_max_rel_l1": err}
+
+
 def _timed_batch(work, precision: int, steps: int):
     """One batch of `work` in `precision`.  Setup (case table, initial
     sequence, autotuning, graph capture and one warm-up replay from the
@@ -524,6 +594,9 @@ def main(argv=None):
                     help="the 3D leg's shock-sphere case has 2n x n x n cells")
     ap.add_argument("--threed-steps", type=int, default=20,
                     help="timed steps of the 3D leg (0 = skip)")
+    ap.add_argument("--threed-batch-cases", type=int, default=32,
+                    help="uneven 3D cases in the 3D batched leg (0: skip)")
+    ap.add_argument("--threed-batch-steps", type=int, default=100)
     ap.add_argument("--late-start", type=int, default=3500,
                     help="also time --steps steps of the headline case from this step on, after "
                          "the shock has crossed the cylinder (0 = skip)")
@@ -658,6 +731,9 @@ def main(argv=None):
     if args.threed_steps > 0:
         out["threed"] = threed_leg(args.threed_n, 3, args.threed_steps)
         _progress(f"3D {out['threed']['value']:.4f} Gcell-updates/s")
+    if args.threed_batch_cases > 0:
+        out["threed_batched"] = threed_batched_leg(args.threed_batch_cases, args.threed_batch_steps)
+        _progress(f"3D batched speedup {out['threed_batched']['speedup']}")
     if args.batch_cases > 0:
         out["batched"] = batched_leg(args.batch_cases, args.batch_steps)
         _progress(f"batched speedup {out['batched']['speedup']}")

@@ -330,6 +330,33 @@ int wamp3_limit_update(wamp3_ctx *ctx, double *const Un[7], double *const U[7],
                        double *const F[48], double *const Uout[7], double alpha, double beta,
                        const double *dt_dev, double *dt_acc, void *stream);
 
+/* Batches of 3D cases (no counterpart in the reference, whose solver is 2D):
+   the wamp_batch_* pattern for the 3D step.  Cases of any grid size and EOS
+   sharing one scheme and precision advance together; each kernel of a stage
+   is one launch over every active case's blocks, and each case's dt, time,
+   end time and done flag stay on the device.  Every case's result is bitwise
+   the result of wamp3_step on that case alone. */
+typedef struct wamp3_batch wamp3_batch;
+int wamp3_batch_create(const wamp_scheme *scheme, int32_t precision, int32_t max_cases,
+                       wamp3_batch **out);
+int wamp3_batch_destroy(wamp3_batch *b);
+/* Put a case in a slot (allocates its scratch: 68 FP64 planes of its padded
+   grid, 47 plane-equivalents in mixed precision); t_end <= 0: no end time.
+   The state U stays the caller's and is advanced in place. */
+int wamp3_batch_set_case(wamp3_batch *b, int32_t slot, const wamp_grid3 *grid,
+                         const wamp_eos *eos, double *const U[7], double t0, double t_end);
+int wamp3_batch_clear_case(wamp3_batch *b, int32_t slot);
+/* Upload the case table and work offsets (after set/clear). */
+int wamp3_batch_commit(wamp3_batch *b, void *stream);
+/* Initial sequence and first dt of the given slots (all occupied if n_slots < 0). */
+int wamp3_batch_prepare(wamp3_batch *b, const int32_t *slots, int32_t n_slots, void *stream);
+/* Capture `steps` steps in a hipGraph that wamp3_batch_step replays (0: no graph). */
+int wamp3_batch_graph(wamp3_batch *b, int32_t steps);
+/* Advance every unfinished case; returns the number of steps enqueued (> 0). */
+int wamp3_batch_step(wamp3_batch *b, void *stream);
+/* t, dt, done of every slot to host arrays of max_cases entries (synchronises). */
+int wamp3_batch_status(wamp3_batch *b, double *t, double *dt, int32_t *done, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

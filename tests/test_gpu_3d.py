@@ -352,3 +352,110 @@ def test_cli_writes_outputs_and_restarts(gpu_ok, tmp_path):
         for k in ("rho", "u", "v", "w", "p", "alpha1"):
             x, y = A[k], B[k]
             assert np.abs(x - y).sum() <= 1e-9 * np.abs(x).sum() + 1e-12, k
+
+
+def _batch3_cases(scheme, precision):
+    """Four uneven 3D cases of one scheme: a shock sphere (24x12x12), the
+    generic cube (14^3), the shock cylinder embedded along z (40x20x3) and a
+    one-cell-thick slab of it (40x1x20)."""
+    import dataclasses
+
+    from wamp import cases, threed
+    from wamp.config import SCHEMES
+
+    sch = SCHEMES[scheme]
+    out = [threed.shock_sphere(6, scheme=sch)]
+    cfg, U = _generic_cube(14)
+    out.append((dataclasses.replace(cfg, scheme=sch), U))
+    cfg2, U2 = cases.shock_cylinder(40, 20, scheme=sch)
+    out.append(threed.from_2d_xy(cfg2, U2, 3))
+    out.append(threed.from_2d_xz(cfg2, U2, 1))
+    return [(dataclasses.replace(c, precision=precision), U) for c, U in out]
+
+
+def _alone3(cfg, U0, steps):
+    """Solver3 on one case: its own CFL dt sequence; returns (state, t)."""
+    from wamp import threed
+
+    s = threed.Solver3(cfg)
+    s.set_state(U0)
+    s.prepare()
+    dts = s.run(steps)
+    out = s.state()
+    s.close()
+    t = 0.0
+    for dt in dts:
+        t += float(dt)
+    return out, t
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("scheme", ["sc", "m10", "mp5"])
+def test_batch3_equals_each_case_alone(gpu_ok, scheme, precision):
+    """Uneven 3D cases in one batch (one launch per kernel and stage over all
+    of them, slots 0 and 5 empty), stepped one by one and by replays of a
+    3-step graph: every case equals Solver3 run alone bit for bit, with the
+    same time."""
+    from wamp import threed
+    from wamp.config import SCHEMES
+
+    cs = _batch3_cases(scheme, precision)
+    steps = 6
+    ref = [_alone3(c, U, steps) for c, U in cs]
+    for graph in (0, 3):
+        b = threed.Batch3(SCHEMES[scheme], precision, max_cases=6)
+        for k, (c, U) in enumerate(cs):
+            b.set_case(k + 1, c, U)
+        b.commit(graph_steps=graph)
+        b.prepare()
+        done = 0
+        while done < steps:
+            done += b.step()
+        assert done == steps
+        t, _, fin = b.status()
+        for k, (c, _) in enumerate(cs):
+            si, sj, sk = c.grid.interior()
+            got, (want, t_want) = b.state(k + 1), ref[k]
+            for m in range(7):
+                assert np.array_equal(got[m][si, sj, sk], want[m][si, sj, sk]), (graph, k, m)
+            assert t[k + 1] == t_want, (graph, k)
+            assert not fin[k + 1]
+        b.close()
+
+
+def test_batch3_end_time_freezes_a_case(gpu_ok):
+    """A case given an end time stops there (the last dt clipped, the case then
+    frozen) while the others go on; a cleared slot can take a new case."""
+    from wamp import threed
+    from wamp.config import SCHEMES
+
+    cs = _batch3_cases("m10", 0)
+    (c0, U0), (c1, U1) = cs[0], cs[2]
+    _, t2 = _alone3(c0, U0, 2)
+    t_end = 1.5 * t2  # inside the third or fourth step
+    b = threed.Batch3(SCHEMES["m10"], 0, max_cases=2)
+    b.set_case(0, c0, U0, t_end=t_end)
+    b.set_case(1, c1, U1)
+    b.commit()
+    b.prepare()
+    for _ in range(8):
+        b.step()
+    t, dt, fin = b.status()
+    assert fin[0] and abs(t[0] - t_end) <= 1e-14 * t_end and dt[0] == 0.0
+    frozen = b.state(0)
+    b.step()
+    assert np.array_equal(b.state(0), frozen)
+    want, t_want = _alone3(c1, U1, 9)
+    si, sj, sk = c1.grid.interior()
+    assert np.array_equal(b.state(1)[:, si, sj, sk], want[:, si, sj, sk])
+    t, _, _ = b.status()
+    assert t[1] == t_want
+    b.clear_case(0)
+    b.set_case(0, cs[1][0], cs[1][1])
+    b.commit()
+    b.prepare([0])
+    b.step()
+    want0, _ = _alone3(cs[1][0], cs[1][1], 1)
+    si, sj, sk = cs[1][0].grid.interior()
+    assert np.array_equal(b.state(0)[:, si, sj, sk], want0[:, si, sj, sk])
+    b.close()

@@ -587,15 +587,17 @@ __device__ __forceinline__ void face3(const Geo3& G, const Eos& e, const State3&
 }
 
 // Faces of sweep DIR: (i, j, k) with the DIR index in [g-1, g+n) and the
-// other two interior; one thread per face, k fastest.
+// other two interior; one thread per face, k fastest.  (bx, by, bz): the
+// block's coordinates in the grid3 block grid of the sweep.
 template <int DIR, int BASE, int SHEAR, int FACEPP, typename T>
-__global__ void __launch_bounds__(256) k3_faces(Geo3 G, Eos e, State3 U, const double* __restrict__ P,
-                                                const double* __restrict__ S,
-                                                const double* __restrict__ D, Faces3 F, int vf_always,
-                                                double t1) {
-  const int k = G.g - (DIR == 2 ? 1 : 0) + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = G.g - (DIR == 1 ? 1 : 0) + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = G.g - (DIR == 0 ? 1 : 0) + (int)blockIdx.z;
+__device__ __forceinline__ void faces3_block(const Geo3& G, const Eos& e, const State3& U,
+                                             const double* __restrict__ P,
+                                             const double* __restrict__ S,
+                                             const double* __restrict__ D, const Faces3& F,
+                                             int vf_always, double t1, int bx, int by, int bz) {
+  const int k = G.g - (DIR == 2 ? 1 : 0) + (int)(bx * 64 + threadIdx.x);
+  const int j = G.g - (DIR == 1 ? 1 : 0) + (int)(by * 4 + threadIdx.y);
+  const int i = G.g - (DIR == 0 ? 1 : 0) + bz;
   if (k >= G.g + G.nz || j >= G.g + G.ny) return;
   double out[16];
   face3<DIR, BASE, SHEAR, FACEPP, T>(G, e, U, P, S, D, vf_always, t1, i, j, k, out);
@@ -603,9 +605,67 @@ __global__ void __launch_bounds__(256) k3_faces(Geo3 G, Eos e, State3 U, const d
 #pragma unroll
   for (int m = 0; m < 16; ++m) fstore3<T>(F, DIR * 16 + m, c, out[m]);
 }
+template <int DIR, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(256) k3_faces(Geo3 G, Eos e, State3 U, const double* __restrict__ P,
+                                                const double* __restrict__ S,
+                                                const double* __restrict__ D, Faces3 F, int vf_always,
+                                                double t1) {
+  faces3_block<DIR, BASE, SHEAR, FACEPP, T>(G, e, U, P, S, D, F, vf_always, t1, (int)blockIdx.x,
+                                            (int)blockIdx.y, (int)blockIdx.z);
+}
 
 inline dim3 grid3(int nk, int nj, int ni) {
   return dim3((unsigned)((nk + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)ni);
+}
+
+// ---------------------------------------------------------- batches -------
+// A batched launch runs the blocks of every active case of the table: its
+// flat block index b falls in case k where off[k] <= b < off[k + 1] (prefix
+// sums of the cases' block counts of one work kind), and the local index
+// b - off[k] is the case's block in its own grid3 block grid.  Every thread
+// of a block belongs to one case, so the block-wide reductions and barriers
+// of the single-case kernels hold.
+__device__ __forceinline__ int find_case3(const int* __restrict__ off, int n, int b) {
+  int lo = 0, hi = n;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+// Case and block coordinates of this block for work kind `kind`; false when
+// the case has finished (the whole block then returns).
+__device__ __forceinline__ bool batch_block3(const Case3Desc* __restrict__ cs,
+                                             const int* __restrict__ off, int ncase,
+                                             const int* __restrict__ done, int kind, int& k,
+                                             int& bx, int& by, int& bz) {
+  const int* o = off + kind * (ncase + 1);
+  const int b = (int)blockIdx.x;
+  k = __builtin_amdgcn_readfirstlane(find_case3(o, ncase, b));
+  if (done[k]) return false;
+  int nk, nj, ni;
+  kind_extent3(cs[k].geo, kind, nk, nj, ni);
+  const int gx = (nk + 63) / 64, gy = (nj + 3) / 4;
+  const int local = b - o[k];
+  bx = local % gx;
+  by = (local / gx) % gy;
+  bz = local / (gx * gy);
+  return true;
+}
+// The faces of sweep DIR of every active case on stage `stage`'s input.
+template <int DIR, int BASE, int SHEAR, int FACEPP, typename T>
+__global__ void __launch_bounds__(256) k3b_faces(const Case3Desc* __restrict__ cs,
+                                                 const int* __restrict__ off, int ncase,
+                                                 BatchScal sc, int stage, int vf_always, double t1) {
+  int k, bx, by, bz;
+  if (!batch_block3(cs, off, ncase, sc.done, B3_FACE0 + DIR, k, bx, by, bz)) return;
+  const Case3Desc& c = cs[k];
+  State3 U;
+#pragma unroll
+  for (int m = 0; m < 7; ++m) U.f[m] = c.u[stage][m];
+  faces3_block<DIR, BASE, SHEAR, FACEPP, T>(c.geo, c.eos, U, c.P, c.S, c.D, c.f, vf_always, t1, bx,
+                                            by, bz);
 }
 
 // The three sweeps of one stage for one scheme.
@@ -621,6 +681,37 @@ inline hipError_t launch_faces3_t(const Geo3& G, const Eos& e, const State3& U, 
   hipLaunchKernelGGL((k3_faces<2, BASE, SHEAR, FACEPP, T>), grid3(G.nz + 1, G.ny, G.nx), b, 0, st,
                      G, e, U, P, S, D, F, vf_always, t1);
   return hipGetLastError();
+}
+// The three sweeps of one stage of a batch; blocks[d]: the table's total
+// block count of sweep d.
+template <int BASE, int SHEAR, int FACEPP, typename T>
+inline hipError_t launch_faces3_batch_t(const Case3Desc* cs, const int* off, int ncase,
+                                        const BatchScal& sc, int stage, int vf_always, double t1,
+                                        const int* blocks, hipStream_t st) {
+  const dim3 b(64, 4);
+  if (blocks[0] > 0)
+    hipLaunchKernelGGL((k3b_faces<0, BASE, SHEAR, FACEPP, T>), dim3(blocks[0]), b, 0, st, cs, off,
+                       ncase, sc, stage, vf_always, t1);
+  if (blocks[1] > 0)
+    hipLaunchKernelGGL((k3b_faces<1, BASE, SHEAR, FACEPP, T>), dim3(blocks[1]), b, 0, st, cs, off,
+                       ncase, sc, stage, vf_always, t1);
+  if (blocks[2] > 0)
+    hipLaunchKernelGGL((k3b_faces<2, BASE, SHEAR, FACEPP, T>), dim3(blocks[2]), b, 0, st, cs, off,
+                       ncase, sc, stage, vf_always, t1);
+  return hipGetLastError();
+}
+template <typename T>
+inline hipError_t launch_faces3_batch(const Case3Desc* cs, const int* off, int ncase,
+                                      const BatchScal& sc, int stage, int base, int shear,
+                                      int vf_always, int face_pp, double t1, const int* blocks,
+                                      hipStream_t st) {
+  if (base == 0 && shear == 1 && face_pp == 0)
+    return launch_faces3_batch_t<0, 1, 0, T>(cs, off, ncase, sc, stage, vf_always, t1, blocks, st);
+  if (base == 0 && shear == 0 && face_pp == 1)
+    return launch_faces3_batch_t<0, 0, 1, T>(cs, off, ncase, sc, stage, vf_always, t1, blocks, st);
+  if (base == 1 && shear == 2 && face_pp == 0)
+    return launch_faces3_batch_t<1, 2, 0, T>(cs, off, ncase, sc, stage, vf_always, t1, blocks, st);
+  return hipErrorInvalidValue;
 }
 template <typename T>
 inline hipError_t launch_faces3(const Geo3& G, const Eos& e, const State3& U, const double* P,

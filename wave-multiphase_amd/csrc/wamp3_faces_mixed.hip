@@ -18,3 +18,12 @@ extern "C" hipError_t wamp3_launch_faces_mixed(const wamp::Geo3* G, const wamp::
   return wamp::mixed::launch_faces3<float>(*G, *e, *U, P, S, D, *F, base, shear, vf_always, face_pp,
                                            t1, st);
 }
+
+extern "C" hipError_t wamp3_launch_faces_batch_mixed(const wamp::Case3Desc* cs, const int* off,
+                                                     int ncase, const wamp::BatchScal* sc,
+                                                     int stage, int base, int shear,
+                                                     int vf_always, int face_pp, double t1,
+                                                     const int* blocks, hipStream_t st) {
+  return wamp::mixed::launch_faces3_batch<float>(cs, off, ncase, *sc, stage, base, shear,
+                                                 vf_always, face_pp, t1, blocks, st);
+}

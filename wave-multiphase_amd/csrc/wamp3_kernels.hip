@@ -80,10 +80,12 @@ __device__ __forceinline__ double ducros3(const Geo3& G, const State3& U, int i,
 // Pressure, entropy (cell_entropy SC:308-314) and, for the shear schemes,
 // the Ducros factor of every cell of the padded grid, ghosts included (a
 // ghost holds the values of its boundary-filled state).
+// (bx, by, bz): the block's coordinates in the grid3 block grid of the padded grid.
 template <int SHEAR>
-__global__ void __launch_bounds__(256) k3_cells(Geo3 G, Eos e, State3 U, double* __restrict__ P,
-                                                double* __restrict__ S, double* __restrict__ D) {
-  const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+__device__ __forceinline__ void cells3_block(const Geo3& G, const Eos& e, const State3& U,
+                                             double* __restrict__ P, double* __restrict__ S,
+                                             double* __restrict__ D, int bx, int by, int bz) {
+  const int k = bx * 64 + (int)threadIdx.x, j = by * 4 + (int)threadIdx.y, i = bz;
   if (k >= G.nz + 2 * G.g || j >= G.ny + 2 * G.g) return;
   double q[7];
   load3(G, U, i, j, k, q);
@@ -93,6 +95,11 @@ __global__ void __launch_bounds__(256) k3_cells(Geo3 G, Eos e, State3 U, double*
   P[c] = p;
   S[c] = entropy(e, q[0], q[1], p, q[6]);
   if (SHEAR != 0) D[c] = ducros3(G, U, i, j, k);
+}
+template <int SHEAR>
+__global__ void __launch_bounds__(256) k3_cells(Geo3 G, Eos e, State3 U, double* __restrict__ P,
+                                                double* __restrict__ S, double* __restrict__ D) {
+  cells3_block<SHEAR>(G, e, U, P, S, D, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // ---------------------------------------------------------- limiter -------
@@ -190,13 +197,13 @@ struct Theta3 {
 // One limiter stage's theta of every interior cell (pp_cell_theta
 // SC:1436-1476), from the fluxes blended by the earlier stages' thetas.
 template <int STAGE, bool F32>
-__global__ void __launch_bounds__(256) k3_theta(Geo3 G, Eos e, State3 U, Faces3 F, Theta3 T,
-                                                const double* __restrict__ dt_dev) {
-  const int k = G.g + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = G.g + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = G.g + (int)blockIdx.z;
+__device__ __forceinline__ void theta3_block(const Geo3& G, const Eos& e, const State3& U,
+                                             const Faces3& F, const Theta3& T, double dt, int bx,
+                                             int by, int bz) {
+  const int k = G.g + bx * 64 + (int)threadIdx.x;
+  const int j = G.g + by * 4 + (int)threadIdx.y;
+  const int i = G.g + bz;
   if (k >= G.g + G.nz || j >= G.g + G.ny) return;
-  const double dt = *dt_dev;
   const double inv[3] = {1.0 / G.dx, 1.0 / G.dy, 1.0 / G.dz};
   const size_t c = at3(G, i, j, k);
   CellFaces3 cf;
@@ -228,6 +235,12 @@ __global__ void __launch_bounds__(256) k3_theta(Geo3 G, Eos e, State3 U, Faces3 
     }
   }
   T.t[STAGE][c] = theta_stage3<STAGE>(e, l, h);
+}
+template <int STAGE, bool F32>
+__global__ void __launch_bounds__(256) k3_theta(Geo3 G, Eos e, State3 U, Faces3 F, Theta3 T,
+                                                const double* __restrict__ dt_dev) {
+  theta3_block<STAGE, F32>(G, e, U, F, T, *dt_dev, (int)blockIdx.x, (int)blockIdx.y,
+                           (int)blockIdx.z);
 }
 
 // enforce_admissible_kernel SC:918-958 with w.
@@ -293,16 +306,16 @@ __device__ __forceinline__ void block_min3(double v, double* acc) {
 // Final fluxes, SSP-RK3 combine (rk3_kernel SC:1592-1611), repair, clamp;
 // with dt_acc the CFL minimum of the new state is folded in.
 template <bool F32>
-__global__ void __launch_bounds__(256) k3_update(Geo3 G, Eos e, State3 Un, State3 U, Faces3 F,
-                                                 Theta3 T, State3 Uo, double alpha, double beta,
-                                                 const double* __restrict__ dt_dev, double* dt_acc,
-                                                 int repair) {
-  const int k = G.g + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = G.g + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = G.g + (int)blockIdx.z;
+__device__ __forceinline__ void update3_block(const Geo3& G, const Eos& e, const State3& Un,
+                                              const State3& U, const Faces3& F, const Theta3& T,
+                                              const State3& Uo, double alpha, double beta,
+                                              double dt, double* dt_acc, int repair, int bx,
+                                              int by, int bz) {
+  const int k = G.g + bx * 64 + (int)threadIdx.x;
+  const int j = G.g + by * 4 + (int)threadIdx.y;
+  const int i = G.g + bz;
   double local_dt = 1.0e30;
   if (k < G.g + G.nz && j < G.g + G.ny) {
-    const double dt = *dt_dev;
     const double inv[3] = {1.0 / G.dx, 1.0 / G.dy, 1.0 / G.dz};
     const size_t c = at3(G, i, j, k);
     CellFaces3 cf;
@@ -322,6 +335,14 @@ __global__ void __launch_bounds__(256) k3_update(Geo3 G, Eos e, State3 Un, State
     if (dt_acc) local_dt = cell_dt3(e, un, G);
   }
   if (dt_acc) block_min3(local_dt, dt_acc);
+}
+template <bool F32>
+__global__ void __launch_bounds__(256) k3_update(Geo3 G, Eos e, State3 Un, State3 U, Faces3 F,
+                                                 Theta3 T, State3 Uo, double alpha, double beta,
+                                                 const double* __restrict__ dt_dev, double* dt_acc,
+                                                 int repair) {
+  update3_block<F32>(G, e, Un, U, F, T, Uo, alpha, beta, *dt_dev, dt_acc, repair, (int)blockIdx.x,
+                     (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // CFL minimum over the interior (timestep_kernel SC:1686-1705).
@@ -356,7 +377,92 @@ __global__ void __launch_bounds__(256) k3_prepare(Geo3 G, Eos e, State3 U, int r
   for (int m = 0; m < 7; ++m) U.f[m][c] = u[m];
 }
 
+// ---------------------------------------------------------- batches -------
+// One launch per kernel and stage over every active case of a table
+// (batch_block3, wamp3_faces.h): the single-case kernels' bodies on each
+// case's own grid, planes, EOS and dt.  Stage s reads U^s (u[s]) and writes
+// u[s + 1] (stage 2: u[0]); the RK coefficients are the step's (SC:2035-2053).
+__device__ __forceinline__ void stage_states3(const Case3Desc& c, int stage, State3& Un,
+                                              State3& U, State3& Uo) {
+#pragma unroll
+  for (int m = 0; m < 7; ++m) {
+    Un.f[m] = c.u[0][m];
+    U.f[m] = c.u[stage][m];
+    Uo.f[m] = c.u[stage == 2 ? 0 : stage + 1][m];
+  }
+}
+template <int SHEAR>
+__global__ void __launch_bounds__(256) k3b_cells(const Case3Desc* __restrict__ cs,
+                                                 const int* __restrict__ off, int ncase,
+                                                 BatchScal sc, int stage) {
+  int k, bx, by, bz;
+  if (!batch_block3(cs, off, ncase, sc.done, B3_CELLS, k, bx, by, bz)) return;
+  const Case3Desc& c = cs[k];
+  State3 Un, U, Uo;
+  stage_states3(c, stage, Un, U, Uo);
+  cells3_block<SHEAR>(c.geo, c.eos, U, c.P, c.S, c.D, bx, by, bz);
+}
+template <int STAGE, bool F32>
+__global__ void __launch_bounds__(256) k3b_theta(const Case3Desc* __restrict__ cs,
+                                                 const int* __restrict__ off, int ncase,
+                                                 BatchScal sc, int stage) {
+  int k, bx, by, bz;
+  if (!batch_block3(cs, off, ncase, sc.done, B3_INT, k, bx, by, bz)) return;
+  const Case3Desc& c = cs[k];
+  State3 Un, U, Uo;
+  stage_states3(c, stage, Un, U, Uo);
+  const Theta3 T{{c.th[0], c.th[1], c.th[2]}};
+  theta3_block<STAGE, F32>(c.geo, c.eos, U, c.f, T, sc.dt[k], bx, by, bz);
+}
+template <bool F32>
+__global__ void __launch_bounds__(256) k3b_update(const Case3Desc* __restrict__ cs,
+                                                  const int* __restrict__ off, int ncase,
+                                                  BatchScal sc, int stage, int repair) {
+  int k, bx, by, bz;
+  if (!batch_block3(cs, off, ncase, sc.done, B3_INT, k, bx, by, bz)) return;
+  const Case3Desc& c = cs[k];
+  State3 Un, U, Uo;
+  stage_states3(c, stage, Un, U, Uo);
+  const Theta3 T{{c.th[0], c.th[1], c.th[2]}};
+  const double alpha = stage == 0 ? 0.0 : stage == 1 ? 0.75 : 1.0 / 3.0;
+  const double beta = stage == 0 ? 1.0 : stage == 1 ? 0.25 : 2.0 / 3.0;
+  update3_block<F32>(c.geo, c.eos, Un, U, c.f, T, Uo, alpha, beta, sc.dt[k],
+                     stage == 2 ? sc.acc + k : nullptr, repair, bx, by, bz);
+}
+// After a step: t += dt; next dt = cfl * min, clipped to the end time
+// (SC:2025-2026); a case that reached its end time is frozen.
+__global__ void k3b_finalize(const Case3Desc* __restrict__ cs, int ncase, BatchScal sc) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncase || sc.done[k]) return;
+  const double t = sc.t[k] + sc.dt[k];
+  sc.t[k] = t;
+  double dt = cs[k].cfl * sc.acc[k];
+  sc.acc[k] = 1.0e30;
+  const double te = sc.tend[k];
+  if (te > 0.0) {
+    if (t >= te - 1.0e-14) {
+      sc.done[k] = 1;
+      dt = 0.0;
+    } else if (t + dt > te) {
+      dt = te - t;
+    }
+  }
+  sc.dt[k] = dt;
+}
+// Initial dt of slot k from its CFL minimum (clipped to the end time).
+__global__ void k3b_dt0(double* acc, BatchScal sc, int k, double cfl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double dt = cfl * *acc;
+    *acc = 1.0e30;
+    const double te = sc.tend[k];
+    if (te > 0.0 && sc.t[k] + dt > te) dt = te - sc.t[k];
+    sc.dt[k] = dt;
+    sc.done[k] = (te > 0.0 && sc.t[k] >= te - 1.0e-14) ? 1 : 0;
+  }
+}
+
 }  // namespace wamp
+
 
 extern "C" {
 using namespace wamp;
@@ -413,8 +519,67 @@ hipError_t wamp3_launch_dt(const Geo3* G, const Eos* e, const State3* U, double*
   return hipGetLastError();
 }
 
+hipError_t wamp3_launch_faces_batch_mixed(const Case3Desc* cs, const int* off, int ncase,
+                                         const BatchScal* sc, int stage, int base, int shear,
+                                         int vf_always, int face_pp, double t1, const int* blocks,
+                                         hipStream_t st);
+
+// One RK stage of a 3D batch: cells, the three sweeps, the three theta
+// stages and the update, each one launch over the table.  blocks[kind]: the
+// table's total block count of each work kind.
+hipError_t wamp3_launch_batch_stage(const Case3Desc* cs, const int* off, int ncase,
+                                    const BatchScal* sc, int stage, int base, int shear,
+                                    int vf_always, int face_pp, int repair, double t1,
+                                    int precision, const int* blocks, hipStream_t st) {
+  const dim3 b(64, 4);
+  if (blocks[B3_CELLS] > 0) {
+    if (shear)
+      hipLaunchKernelGGL(k3b_cells<1>, dim3(blocks[B3_CELLS]), b, 0, st, cs, off, ncase, *sc, stage);
+    else
+      hipLaunchKernelGGL(k3b_cells<0>, dim3(blocks[B3_CELLS]), b, 0, st, cs, off, ncase, *sc, stage);
+  }
+  hipError_t e = precision == 0
+                     ? launch_faces3_batch<double>(cs, off, ncase, *sc, stage, base, shear,
+                                                   vf_always, face_pp, t1, blocks + B3_FACE0, st)
+                     : wamp3_launch_faces_batch_mixed(cs, off, ncase, sc, stage, base, shear,
+                                                      vf_always, face_pp, t1, blocks + B3_FACE0, st);
+  if (e != hipSuccess) return e;
+  const int nb = blocks[B3_INT];
+  if (nb <= 0) return hipGetLastError();
+  if (precision) {
+    hipLaunchKernelGGL((k3b_theta<0, true>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL((k3b_theta<1, true>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL((k3b_theta<2, true>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL(k3b_update<true>, dim3(nb), b, 0, st, cs, off, ncase, *sc, stage, repair);
+  } else {
+    hipLaunchKernelGGL((k3b_theta<0, false>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL((k3b_theta<1, false>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL((k3b_theta<2, false>), dim3(nb), b, 0, st, cs, off, ncase, *sc, stage);
+    hipLaunchKernelGGL(k3b_update<false>, dim3(nb), b, 0, st, cs, off, ncase, *sc, stage, repair);
+  }
+  return hipGetLastError();
+}
+
+hipError_t wamp3_launch_batch_finalize(const Case3Desc* cs, int ncase, const BatchScal* sc,
+                                       hipStream_t st) {
+  hipLaunchKernelGGL(k3b_finalize, dim3((ncase + 63) / 64), dim3(64), 0, st, cs, ncase, *sc);
+  return hipGetLastError();
+}
+
+// Initial sequence of slot k: repair and clamp, then its first dt.
+hipError_t wamp3_launch_batch_prepare(const Geo3* G, const Eos* e, const State3* U, int repair,
+                                      double* acc, const BatchScal* sc, int k, double cfl,
+                                      hipStream_t st) {
+  const dim3 g = grid3(G->nz, G->ny, G->nx);
+  hipLaunchKernelGGL(k3_prepare, g, dim3(64, 4), 0, st, *G, *e, *U, repair);
+  hipLaunchKernelGGL(k3_dt, g, dim3(64, 4), 0, st, *G, *e, *U, acc);
+  hipLaunchKernelGGL(k3b_dt0, dim3(1), dim3(64), 0, st, acc, *sc, k, cfl);
+  return hipGetLastError();
+}
+
 hipError_t wamp3_launch_prepare(const Geo3* G, const Eos* e, const State3* U, int repair,
                                 hipStream_t st) {
+
   hipLaunchKernelGGL(k3_prepare, grid3(G->nz, G->ny, G->nx), dim3(64, 4), 0, st, *G, *e, *U,
                      repair);
   return hipGetLastError();

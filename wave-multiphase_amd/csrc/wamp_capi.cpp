@@ -83,6 +83,15 @@ hipError_t wamp3_launch_limit_update(const Geo3*, const Eos*, const State3* Un, 
                                      double* dt_acc, int repair, hipStream_t);
 hipError_t wamp3_launch_dt(const Geo3*, const Eos*, const State3*, double* dt_acc, hipStream_t);
 hipError_t wamp3_launch_prepare(const Geo3*, const Eos*, const State3*, int repair, hipStream_t);
+hipError_t wamp3_launch_batch_stage(const Case3Desc* cs, const int* off, int ncase,
+                                    const BatchScal* sc, int stage, int base, int shear,
+                                    int vf_always, int face_pp, int repair, double t1,
+                                    int precision, const int* blocks, hipStream_t st);
+hipError_t wamp3_launch_batch_finalize(const Case3Desc* cs, int ncase, const BatchScal* sc,
+                                       hipStream_t st);
+hipError_t wamp3_launch_batch_prepare(const Geo3* G, const Eos* e, const State3* U, int repair,
+                                      double* acc, const BatchScal* sc, int k, double cfl,
+                                      hipStream_t st);
 }
 
 // A context is a step plan plus the scratch it owns.
@@ -965,11 +974,10 @@ static int state3_of(double* const U[7], State3& s, const char* name) {
   return 0;
 }
 
-int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_scheme* scheme,
-                     int32_t precision, wamp3_ctx** out) {
-  if (!out) return fail(-1, "out is NULL");
-  *out = nullptr;
-  if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
+}  // extern "C"
+
+// Checks of a 3D grid and EOS shared by contexts and batches; 0 when valid.
+static int check_grid3(const wamp_grid3* grid, const wamp_eos* eos) {
   if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1)
     return fail(-3, "nx, ny, nz must be >= 1 (got %d, %d, %d)", grid->nx, grid->ny, grid->nz);
   if (grid->g < 4) return fail(-3, "g must be >= 4 (the face stencil reaches 4 cells), got %d", grid->g);
@@ -983,17 +991,18 @@ int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_sch
   for (int k = 0; k < 6; ++k)
     if (grid->bc[k] != 0 && grid->bc[k] != 1) return fail(-3, "bc[%d] must be 0 or 1", k);
   if (!(eos->gamma1 > 1.0) || !(eos->gamma2 > 1.0)) return fail(-3, "gamma must be > 1");
+  return 0;
+}
+static int check_scheme3(const wamp_scheme* scheme) {
   if (scheme->vf_thinc < 0 || scheme->vf_thinc > 1) return fail(-3, "scheme.vf_thinc must be 0 or 1");
   if (!wamp3_scheme_available(scheme->base, scheme->shear, scheme->face_pp))
     return fail(-4, "3D: scheme (base %d, shear %d, face_pp %d) is not built; available: SC (0,1,0), "
                 "M10 (0,0,1), MP5 (1,2,0)", scheme->base, scheme->shear, scheme->face_pp);
   if (!(scheme->cfl > 0.0)) return fail(-3, "scheme.cfl must be > 0");
-  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
-    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
-
-  wamp3_ctx* c = new (std::nothrow) wamp3_ctx();
-  if (!c) return fail(-5, "host allocation failed");
-  Geo3& G = c->geo;
+  return 0;
+}
+static Geo3 make_geo3(const wamp_grid3* grid) {
+  Geo3 G;
   G.nx = grid->nx;
   G.ny = grid->ny;
   G.nz = grid->nz;
@@ -1004,24 +1013,52 @@ int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_sch
   G.dy = grid->dy;
   G.dz = grid->dz;
   for (int k = 0; k < 6; ++k) G.bc[k] = grid->bc[k];
+  return G;
+}
+static size_t plane3(const wamp_grid3* grid) {
+  return (size_t)(grid->nx + 2 * grid->g) * (grid->ny + 2 * grid->g) * grid->ldz;
+}
+// face planes: 48 FP64, or in mixed precision 6 FP64 energy planes and 42
+// FP32 planes (a multiple of 8 bytes: 42 * 4 * plane)
+static size_t faces3_bytes(size_t plane, int precision) {
+  return precision == 0 ? 48 * plane * sizeof(double)
+                        : 6 * plane * sizeof(double) + 42 * plane * sizeof(float);
+}
+static Faces3 carve_faces3(char* base, size_t plane, int precision) {
+  Faces3 f{(double*)base, (long long)plane};
+  if (precision != 0) {
+    f.h = (float*)(base + 6 * plane * sizeof(double));
+    f.hs = (long long)plane;
+  }
+  return f;
+}
+
+extern "C" {
+
+int wamp3_ctx_create(const wamp_grid3* grid, const wamp_eos* eos, const wamp_scheme* scheme,
+                     int32_t precision, wamp3_ctx** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (!grid || !eos || !scheme) return fail(-1, "grid, eos and scheme must be non-NULL");
+  if (int rc = check_grid3(grid, eos)) return rc;
+  if (int rc = check_scheme3(scheme)) return rc;
+  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
+    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
+
+  wamp3_ctx* c = new (std::nothrow) wamp3_ctx();
+  if (!c) return fail(-5, "host allocation failed");
+  c->geo = make_geo3(grid);
   c->eos = make_eos(*eos);
   c->scheme = *scheme;
   c->precision = precision;
   c->t1 = wm_tanh(1.8 * 0.5);
-  const size_t plane = (size_t)plane64;
-  // face planes: 48 FP64, or in mixed precision 6 FP64 energy planes and 42
-  // FP32 planes (a multiple of 8 bytes: 42 * 4 * plane)
-  const size_t fbytes = precision == 0 ? 48 * plane * sizeof(double)
-                                       : 6 * plane * sizeof(double) + 42 * plane * sizeof(float);
+  const size_t plane = plane3(grid);
+  const size_t fbytes = faces3_bytes(plane, precision);
   const size_t bytes = fbytes + ((14 + 3 + 3) * plane + 32) * sizeof(double);
   int rc = hip_check(hipMalloc((void**)&c->block, bytes), "hipMalloc(3D scratch)");
   if (rc) { delete c; return rc; }
   c->bytes = (int64_t)bytes;
-  c->faces = Faces3{c->block, (long long)plane};
-  if (precision != 0) {
-    c->faces.h = (float*)((char*)c->block + 6 * plane * sizeof(double));
-    c->faces.hs = (long long)plane;
-  }
+  c->faces = carve_faces3((char*)c->block, plane, precision);
   double* p = (double*)((char*)c->block + fbytes);
   for (int m = 0; m < 7; ++m, p += plane) c->u1.f[m] = p;
   for (int m = 0; m < 7; ++m, p += plane) c->u2.f[m] = p;
@@ -1147,6 +1184,281 @@ int wamp3_step(wamp3_ctx* c, double* const U[7], const double* dt_dev, double* d
       return rc;
   }
   return hip_check(wamp_launch_dt_finalize(c->dt_acc, dt_next_dev, sc.cfl, st), "3D dt");
+}
+
+// ---- batches of 3D cases ---------------------------------------------------
+// Slots of resident 3D cases sharing one scheme and precision.  Each step is
+// one launch per kernel and stage over every active case's blocks, with
+// per-case dt, time, end time and done flag on the device (as wamp_batch).
+struct wamp3_batch {
+  wamp_scheme scheme;
+  int precision;
+  int max_cases;
+  double t1;
+  std::vector<Case3Desc> desc;
+  std::vector<int> used;
+  std::vector<char*> scratch;
+  std::vector<double> t0, tend;
+  Case3Desc* desc_dev;
+  int* off_dev;  // B3_KINDS prefix arrays of max_cases + 1 block offsets
+  double* scal;  // dt | acc | t | tend
+  int* done;
+  BatchScal sc;
+  int committed;
+  int blocks[B3_KINDS];
+  hipStream_t cap_stream;
+  hipGraphExec_t graph;
+  int graph_steps;
+};
+
+static void graph3_drop(wamp3_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+int wamp3_batch_create(const wamp_scheme* scheme, int32_t precision, int32_t max_cases,
+                       wamp3_batch** out) {
+  if (!out) return fail(-1, "out is NULL");
+  *out = nullptr;
+  if (!scheme) return fail(-1, "scheme is NULL");
+  if (max_cases < 1 || max_cases > 4096) return fail(-3, "max_cases must be in [1, 4096]");
+  if (precision != WAMP_PREC_FP64 && precision != WAMP_PREC_MIXED)
+    return fail(-4, "precision %d is not available (0 = FP64, 1 = mixed)", precision);
+  if (int rc = check_scheme3(scheme)) return rc;
+  wamp3_batch* b = new (std::nothrow) wamp3_batch();
+  if (!b) return fail(-5, "host allocation failed");
+  b->scheme = *scheme;
+  b->precision = precision;
+  b->max_cases = max_cases;
+  b->t1 = wm_tanh(1.8 * 0.5);
+  b->desc.assign(max_cases, Case3Desc());
+  b->used.assign(max_cases, 0);
+  b->scratch.assign(max_cases, nullptr);
+  b->t0.assign(max_cases, 0.0);
+  b->tend.assign(max_cases, 0.0);
+  int rc = hip_check(hipMalloc((void**)&b->desc_dev, sizeof(Case3Desc) * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->off_dev, sizeof(int) * B3_KINDS * (max_cases + 1)), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->scal, sizeof(double) * 4 * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->done, sizeof(int) * max_cases), "hipMalloc");
+  if (!rc) rc = hip_check(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (!rc) {
+    b->sc.dt = b->scal;
+    b->sc.acc = b->scal + max_cases;
+    b->sc.t = b->scal + 2 * max_cases;
+    b->sc.tend = b->scal + 3 * max_cases;
+    b->sc.done = b->done;
+    std::vector<double> init(4 * max_cases, 0.0);
+    for (int k = 0; k < max_cases; ++k) init[max_cases + k] = 1.0e30;
+    std::vector<int> ones(max_cases, 1);
+    rc = hip_check(hipMemcpy(b->scal, init.data(), sizeof(double) * init.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = hip_check(hipMemcpy(b->done, ones.data(), sizeof(int) * max_cases, hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  if (rc) {
+    (void)hipFree(b->desc_dev);
+    (void)hipFree(b->off_dev);
+    (void)hipFree(b->scal);
+    (void)hipFree(b->done);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int wamp3_batch_destroy(wamp3_batch* b) {
+  if (!b) return 0;
+  graph3_drop(b);
+  for (char* p : b->scratch)
+    if (p) (void)hipFree(p);
+  (void)hipFree(b->desc_dev);
+  (void)hipFree(b->off_dev);
+  (void)hipFree(b->scal);
+  (void)hipFree(b->done);
+  (void)hipStreamDestroy(b->cap_stream);
+  delete b;
+  return 0;
+}
+
+// Put a 3D case in a slot (allocates its scratch: the face planes, two stage
+// buffers, pressure / entropy / Ducros and the three theta planes).
+int wamp3_batch_set_case(wamp3_batch* b, int32_t slot, const wamp_grid3* grid,
+                         const wamp_eos* eos, double* const U[7], double t0, double t_end) {
+  if (!b) return fail(-1, "NULL batch");
+  if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
+  if (!grid || !eos) return fail(-1, "grid and eos must be non-NULL");
+  if (int rc = check_grid3(grid, eos)) return rc;
+  State3 s;
+  if (int rc = state3_of(U, s, "U")) return rc;
+  const size_t plane = plane3(grid);
+  const size_t fbytes = faces3_bytes(plane, b->precision);
+  char* base = nullptr;
+  if (int rc = hip_check(hipMalloc((void**)&base, fbytes + (14 + 3 + 3) * plane * sizeof(double)),
+                         "hipMalloc(3D case scratch)"))
+    return rc;
+  if (b->scratch[slot]) {
+    (void)hipDeviceSynchronize();  // enqueued work of the batch may still use it
+    (void)hipFree(b->scratch[slot]);
+  }
+  b->scratch[slot] = base;
+  graph3_drop(b);
+  Case3Desc& d = b->desc[slot];
+  d = Case3Desc();
+  d.geo = make_geo3(grid);
+  d.eos = make_eos(*eos);
+  for (int m = 0; m < 7; ++m) d.u[0][m] = U[m];
+  d.f = carve_faces3(base, plane, b->precision);
+  double* p = (double*)(base + fbytes);
+  for (int m = 0; m < 7; ++m, p += plane) d.u[1][m] = p;
+  for (int m = 0; m < 7; ++m, p += plane) d.u[2][m] = p;
+  d.P = p; p += plane;
+  d.S = p; p += plane;
+  d.D = p; p += plane;
+  for (int q = 0; q < 3; ++q, p += plane) d.th[q] = p;
+  d.cfl = b->scheme.cfl;
+  b->used[slot] = 1;
+  b->t0[slot] = t0;
+  b->tend[slot] = t_end;
+  b->committed = 0;
+  return 0;
+}
+
+int wamp3_batch_clear_case(wamp3_batch* b, int32_t slot) {
+  if (!b) return fail(-1, "NULL batch");
+  if (slot < 0 || slot >= b->max_cases) return fail(-3, "slot %d out of range", slot);
+  if (b->scratch[slot]) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->scratch[slot]);
+  }
+  b->scratch[slot] = nullptr;
+  b->used[slot] = 0;
+  b->desc[slot] = Case3Desc();
+  b->committed = 0;
+  return 0;
+}
+
+// Upload the case table and the per-kind block offsets.
+int wamp3_batch_commit(wamp3_batch* b, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  hipStream_t st = (hipStream_t)stream;
+  graph3_drop(b);
+  const int n = b->max_cases;
+  std::vector<int> off(B3_KINDS * (n + 1), 0);
+  for (int kind = 0; kind < B3_KINDS; ++kind) {
+    int* o = off.data() + kind * (n + 1);
+    long long acc = 0;
+    for (int k = 0; k < n; ++k) {
+      if (b->used[k]) acc += kind_blocks3(b->desc[k].geo, kind);
+      if (acc > (1LL << 31) - 1) return fail(-3, "batch too large (more than 2^31 blocks)");
+      o[k + 1] = (int)acc;
+    }
+    b->blocks[kind] = o[n];
+  }
+  if (int rc = hip_check(hipMemcpyAsync(b->desc_dev, b->desc.data(), sizeof(Case3Desc) * n,
+                                        hipMemcpyHostToDevice, st), "hipMemcpyAsync(desc)")) return rc;
+  if (int rc = hip_check(hipMemcpyAsync(b->off_dev, off.data(), sizeof(int) * off.size(),
+                                        hipMemcpyHostToDevice, st), "hipMemcpyAsync(offsets)")) return rc;
+  if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+  b->committed = 1;
+  return 0;
+}
+
+// Initial sequence (repair + clamp) and first dt of the listed slots (all
+// occupied slots when n_slots < 0); sets t = t0.
+int wamp3_batch_prepare(wamp3_batch* b, const int32_t* slots, int32_t n_slots, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int> list;
+  if (n_slots < 0) {
+    for (int k = 0; k < b->max_cases; ++k)
+      if (b->used[k]) list.push_back(k);
+  } else {
+    if (!slots && n_slots > 0) return fail(-2, "slots is NULL");
+    for (int q = 0; q < n_slots; ++q) {
+      if (slots[q] < 0 || slots[q] >= b->max_cases || !b->used[slots[q]])
+        return fail(-3, "slot %d is not occupied", slots[q]);
+      list.push_back(slots[q]);
+    }
+  }
+  for (int k : list) {
+    const Case3Desc& d = b->desc[k];
+    State3 s;
+    for (int m = 0; m < 7; ++m) s.f[m] = d.u[0][m];
+    double vals[2] = {b->t0[k], b->tend[k]};
+    if (int rc = hip_check(hipMemcpyAsync(b->sc.t + k, &vals[0], sizeof(double), hipMemcpyHostToDevice, st), "t0")) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(b->sc.tend + k, &vals[1], sizeof(double), hipMemcpyHostToDevice, st), "tend")) return rc;
+    if (int rc = hip_check(wamp3_launch_batch_prepare(&d.geo, &d.eos, &s, b->scheme.repair,
+                                                      b->sc.acc + k, &b->sc, k, d.cfl, st),
+                           "3D batch prepare"))
+      return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;  // vals lifetime
+  }
+  return 0;
+}
+
+static int batch3_enqueue_step(wamp3_batch* b, hipStream_t st) {
+  const wamp_scheme& sc = b->scheme;
+  for (int s = 0; s < 3; ++s)
+    if (int rc = hip_check(wamp3_launch_batch_stage(b->desc_dev, b->off_dev, b->max_cases, &b->sc, s,
+                                                    sc.base, sc.shear, sc.vf_thinc, sc.face_pp,
+                                                    sc.repair, b->t1, b->precision, b->blocks, st),
+                           "3D batch stage"))
+      return rc;
+  return hip_check(wamp3_launch_batch_finalize(b->desc_dev, b->max_cases, &b->sc, st),
+                   "3D batch finalize");
+}
+
+// Capture `steps` consecutive steps in a hipGraph; 0 drops the graph.
+int wamp3_batch_graph(wamp3_batch* b, int32_t steps) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  graph3_drop(b);
+  if (steps <= 0) return 0;
+  hipGraph_t g = nullptr;
+  if (int rc = hip_check(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal),
+                         "hipStreamBeginCapture")) return rc;
+  int rc = 0;
+  for (int k = 0; k < steps && !rc; ++k) rc = batch3_enqueue_step(b, b->cap_stream);
+  int rc2 = hip_check(hipStreamEndCapture(b->cap_stream, &g), "hipStreamEndCapture");
+  if (rc || rc2) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : rc2;
+  }
+  rc = hip_check(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  if (rc) {
+    b->graph = nullptr;
+    return rc;
+  }
+  b->graph_steps = steps;
+  return 0;
+}
+
+// Advance every occupied, unfinished case: with a captured graph, one replay
+// (graph_steps steps); otherwise one step.  Returns the steps enqueued (> 0).
+int wamp3_batch_step(wamp3_batch* b, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  if (!b->committed) return fail(-6, "batch not committed");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->graph) {
+    if (int rc = hip_check(hipGraphLaunch(b->graph, st), "hipGraphLaunch")) return rc;
+    return b->graph_steps;
+  }
+  if (int rc = batch3_enqueue_step(b, st)) return rc;
+  return 1;
+}
+
+// Copy t, dt and done of every slot to the host (synchronises the stream).
+int wamp3_batch_status(wamp3_batch* b, double* t, double* dt, int32_t* done, void* stream) {
+  if (!b) return fail(-1, "NULL batch");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = b->max_cases;
+  if (t && hip_check(hipMemcpyAsync(t, b->sc.t, sizeof(double) * n, hipMemcpyDeviceToHost, st), "t")) return -7;
+  if (dt && hip_check(hipMemcpyAsync(dt, b->sc.dt, sizeof(double) * n, hipMemcpyDeviceToHost, st), "dt")) return -7;
+  if (done && hip_check(hipMemcpyAsync(done, b->done, sizeof(int) * n, hipMemcpyDeviceToHost, st), "done")) return -7;
+  return hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
 }  // extern "C"

@@ -111,4 +111,36 @@ struct BatchScal {
   int* done;
 };
 
+// One case of a 3D batch (device-resident table).  Its work is counted in
+// blocks of the 3D kernels' (64, 4) thread shape, per kind: the padded grid
+// (cells), the faces of each sweep, the interior (limiter and update).
+enum { B3_CELLS = 0, B3_FACE0 = 1, B3_INT = 4, B3_KINDS = 5 };
+struct Case3Desc {
+  Geo3 geo;
+  Eos eos;
+  double* u[3][7];  // [0] state (U^n, overwritten by stage 3), [1] U^1, [2] U^2
+  double *P, *S, *D;  // pressure, entropy, Ducros planes
+  Faces3 f;
+  double* th[3];    // limiter theta planes
+  double cfl;
+};
+// (nk, nj, ni) extents of work kind `kind` on grid G; its blocks are
+// ceil(nk / 64) x ceil(nj / 4) x ni.
+__host__ __device__ inline void kind_extent3(const Geo3& G, int kind, int& nk, int& nj, int& ni) {
+  if (kind == B3_CELLS) {
+    nk = G.nz + 2 * G.g;
+    nj = G.ny + 2 * G.g;
+    ni = G.nx + 2 * G.g;
+  } else {
+    nk = G.nz + (kind == B3_FACE0 + 2 ? 1 : 0);
+    nj = G.ny + (kind == B3_FACE0 + 1 ? 1 : 0);
+    ni = G.nx + (kind == B3_FACE0 ? 1 : 0);
+  }
+}
+__host__ __device__ inline long long kind_blocks3(const Geo3& G, int kind) {
+  int nk, nj, ni;
+  kind_extent3(G, kind, nk, nj, ni);
+  return (long long)((nk + 63) / 64) * ((nj + 3) / 4) * ni;
+}
+
 }  // namespace wamp

@@ -115,6 +115,17 @@ SIGNATURES = [
     ("wamp3_prepare", C.c_int, [_VP, PtrArr7, _VP]),
     ("wamp3_get_dt", C.c_int, [_VP, PtrArr7, _VP, _VP]),
     ("wamp3_step", C.c_int, [_VP, PtrArr7, _VP, _VP, _VP]),
+    ("wamp3_batch_create", C.c_int, [C.POINTER(WampScheme), C.c_int32, C.c_int32, C.POINTER(_VP)]),
+    ("wamp3_batch_destroy", C.c_int, [_VP]),
+    ("wamp3_batch_set_case", C.c_int, [_VP, C.c_int32, C.POINTER(WampGrid3), C.POINTER(WampEos),
+                                       PtrArr7, C.c_double, C.c_double]),
+    ("wamp3_batch_clear_case", C.c_int, [_VP, C.c_int32]),
+    ("wamp3_batch_commit", C.c_int, [_VP, _VP]),
+    ("wamp3_batch_prepare", C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, _VP]),
+    ("wamp3_batch_graph", C.c_int, [_VP, C.c_int32]),
+    ("wamp3_batch_step", C.c_int, [_VP, _VP]),
+    ("wamp3_batch_status", C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int32), _VP]),
     ("wamp3_compute_fluxes", C.c_int, [_VP, PtrArr7, PtrArr48, _VP]),
     ("wamp3_limit_update", C.c_int, [_VP, PtrArr7, PtrArr7, PtrArr48, PtrArr7, C.c_double,
                                      C.c_double, _VP, _VP, _VP]),

@@ -336,6 +336,103 @@ class Solver3:
         return dts.cpu().numpy()
 
 
+class Batch3:
+    """Slots of resident 3D cases sharing one scheme and precision (libwamp's
+    wamp3_batch_*): each kernel of a stage is one launch over every active
+    case, and each case's dt, time, end time and done flag stay on the device.
+    Every case advances bit for bit as Solver3 does on it alone."""
+
+    def __init__(self, scheme, precision: int, max_cases: int, device: str = "cuda"):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise _lib.WampError("batches need a ROCm GPU")
+        self.L = _lib.load()
+        self.device = torch.device(device)
+        self.scheme = scheme
+        self.precision = int(precision)
+        self.max_cases = int(max_cases)
+        sch = _lib.WampScheme(scheme.base, scheme.shear, scheme.vf_thinc, scheme.face_pp,
+                              1 if scheme.repair else 0, scheme.cfl)
+        h = C.c_void_p()
+        _lib.check(self.L.wamp3_batch_create(C.byref(sch), self.precision, self.max_cases,
+                                             C.byref(h)), "wamp3_batch_create")
+        self.h = h
+        self.cfgs: dict[int, Case3Config] = {}
+        self.states: dict[int, object] = {}
+        self.graph_steps = 0
+
+    @property
+    def stream(self) -> int:
+        return _torch().cuda.current_stream(self.device).cuda_stream
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.wamp3_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_case(self, slot: int, cfg: Case3Config, U0, t0: float = 0.0, t_end: float = 0.0):
+        torch = _torch()
+        g = cfg.grid
+        if cfg.precision != self.precision or cfg.scheme != self.scheme:
+            raise ValueError("a batch holds one scheme and one precision")
+        U = torch.zeros((7, g.nxg, g.nyg, g.nzg), dtype=torch.float64, device=self.device)
+        src = U0 if isinstance(U0, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(U0, dtype=np.float64))
+        U.copy_(src[..., : g.nzg])
+        grid = _lib.WampGrid3(g.nx, g.ny, g.nz, g.g, g.nzg, g.dx, g.dy, g.dz, (C.c_int32 * 6)(*g.bc))
+        eos = _lib.WampEos(*cfg.eos.as_tuple())
+        ptrs = _lib.PtrArr7(*[U[m].data_ptr() for m in range(7)])
+        _lib.check(self.L.wamp3_batch_set_case(self.h, int(slot), C.byref(grid), C.byref(eos), ptrs,
+                                               float(t0), float(t_end)), "wamp3_batch_set_case")
+        self.cfgs[slot] = cfg
+        self.states[slot] = U
+
+    def clear_case(self, slot: int):
+        _lib.check(self.L.wamp3_batch_clear_case(self.h, int(slot)), "wamp3_batch_clear_case")
+        self.cfgs.pop(slot, None)
+        self.states.pop(slot, None)
+
+    def state(self, slot: int) -> np.ndarray:
+        return self.states[slot].cpu().numpy().copy()
+
+    def commit(self, graph_steps: int = 0):
+        _lib.check(self.L.wamp3_batch_commit(self.h, self.stream), "wamp3_batch_commit")
+        self.graph_steps = 0
+        if graph_steps > 0:
+            self.capture(graph_steps)
+
+    def capture(self, steps: int):
+        _torch().cuda.synchronize(self.device)
+        _lib.check(self.L.wamp3_batch_graph(self.h, int(steps)), "wamp3_batch_graph")
+        self.graph_steps = int(steps)
+
+    def prepare(self, slots=None):
+        if slots is None:
+            _lib.check(self.L.wamp3_batch_prepare(self.h, None, -1, self.stream), "wamp3_batch_prepare")
+        else:
+            arr = (C.c_int32 * len(slots))(*slots)
+            _lib.check(self.L.wamp3_batch_prepare(self.h, arr, len(slots), self.stream),
+                       "wamp3_batch_prepare")
+
+    def step(self) -> int:
+        rc = self.L.wamp3_batch_step(self.h, self.stream)
+        if rc <= 0:
+            _lib.check(rc if rc != 0 else -1, "wamp3_batch_step")
+        return rc
+
+    def status(self):
+        n = self.max_cases
+        t, dt, done = (C.c_double * n)(), (C.c_double * n)(), (C.c_int32 * n)()
+        _lib.check(self.L.wamp3_batch_status(self.h, t, dt, done, self.stream), "wamp3_batch_status")
+        return np.array(t), np.array(dt), np.array(done, dtype=bool)
+
+
 def main(argv=None) -> int:
     """python -m wamp.threed: run the 3D shock-sphere case and write npz outputs."""
     import argparse
