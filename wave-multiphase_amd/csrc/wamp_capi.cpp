@@ -383,11 +383,13 @@ static int lu_rows_lo(const StepPlan* c) {
 // phases) has the lower latency on small grids, whose march would leave most
 // SIMDs idle; the marching kernel moves fewer bytes and wins from about
 // 2^18 cells (measured: 512x256 0.12 vs 0.19 ms per step, 1024x512 0.37 vs
-// 0.33, 2048x1024 1.45 vs 1.14).  Kind 2 is the march that forms the
-// first-order fluxes itself (FP64 face storage only), so the face kernel
-// skips them; it is the FP64 step's march (4096x2048 M10: faces 1.07 instead
-// of 1.27 ms, limiter 0.97 instead of 0.85 ms per launch, step 6.12 instead
-// of 6.36 ms).  Every kind gives bitwise the same result.  `planes`: the
+// 0.33, 2048x1024 1.45 vs 1.14; with the one-round rows per wave the
+// marches win from 1024x512 on and the LDS-tile kernel up to 512x256).
+// Kind 2 is the march that forms the first-order fluxes itself (FP64 face
+// storage only), so the face kernel skips them; it is the FP64 step's march
+// (4096x2048 M10, per launch: faces 1.02 instead of 1.27 ms with the
+// first-order planes, limiter 0.665 ms).  Every kind gives bitwise the same
+// result.  `planes`: the
 // caller hands over all 28 face planes (the split entry point), which kind 2
 // reads only when WAMP_LU_KIND asks for it.
 static int lu_kind(const StepPlan* c, bool planes = false) {
