@@ -231,7 +231,8 @@ int wamp_batch_set_case(wamp_batch *b, int32_t slot, const wamp_grid *grid, cons
 int wamp_batch_clear_case(wamp_batch *b, int32_t slot);
 /* Work decomposition: tile 0 = 8x32 face tiles, 1 = 16x16; persistent blocks
    per CU for the face kernel (1..16) and the limiter kernel (0 = one wave per
-   limiter item, 1..16 persistent); limiter rows per wave (4..4096).
+   limiter item, 1..16 persistent); limiter rows per wave (4..4096, or 0:
+   chosen at each commit for the resident cases, the fewest rounds x fronts).
    Results do not depend on these. */
 int wamp_batch_set_tuning(wamp_batch *b, int32_t tile, int32_t faces_blocks_per_cu,
                           int32_t lu_blocks_per_cu, int32_t lu_seg);
@@ -247,7 +248,8 @@ int wamp_batch_graph(wamp_batch *b, int32_t steps);
 int wamp_batch_step(wamp_batch *b, void *stream);
 /* Per-slot time, dt and finished flag to the host (synchronises the stream). */
 int wamp_batch_status(wamp_batch *b, double *t, double *dt, int32_t *done, void *stream);
-/* Work-list lengths: which = 0 face tiles, 1 limiter wave items. */
+/* Work-list lengths: which = 0 face tiles, 1 limiter wave items; 2: the
+   limiter rows per wave in effect. */
 int64_t wamp_batch_work(const wamp_batch *b, int32_t which);
 
 /* Measurement hook (no counterpart in the reference; for benchmarks and

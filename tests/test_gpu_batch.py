@@ -43,12 +43,15 @@ def test_batch_equals_alone_under_all_tunings(gpu_ok, precision):
     # striding over the items
     for tuning, graph in ((Tuning(0, 3, 2, 16), 0), (Tuning(1, 2, 4, 8), 3), (Tuning(0, 1, 1, 32), 6),
                           (Tuning(1, 3, 2, 26), 2), (Tuning(0, 3, 0, 20), 0), (Tuning(1, 3, 0, 26), 2),
-                          (Tuning(0, 3, 0, 5), 0), (Tuning(1, 2, 2, 100), 2)):
+                          (Tuning(0, 3, 0, 5), 0), (Tuning(1, 2, 2, 100), 2),
+                          (Tuning(0, 3, 0, 0), 0), (Tuning(1, 3, 2, 0), 3)):
         b = Batch(cs[0][0].scheme, precision, max_cases=6)
         for k, (c, U) in enumerate(cs):
             b.set_case(k + 1, c, U)  # slot 0 and 5 stay empty
         b.set_tuning(tuning)
         b.commit()
+        rows = b.work()[2]
+        assert rows == tuning.lu_seg if tuning.lu_seg else 4 <= rows <= 4096
         b.prepare()
         if graph:
             b.capture(graph)

@@ -614,7 +614,8 @@ struct wamp_batch {
   int tile;            // 0: 8x32 face tiles, 1: 16x16
   int faces_bpcu;      // persistent face blocks per CU
   int lu_bpcu;         // persistent limiter blocks (4 waves) per CU; 0: one wave per item
-  int lu_seg;          // limiter rows per wave
+  int lu_seg;          // limiter rows per wave (in effect)
+  int lu_seg_req;      // rows per wave asked for; 0: chosen at each commit
   // host mirror of the case table
   std::vector<CaseDesc> desc;
   std::vector<int> used;         // slot occupied
@@ -644,6 +645,41 @@ static void case_work(wamp_batch* b, CaseDesc& d) {
   d.tiles = d.tiles_j * ((d.geo.nx + 1 + ti - 1) / ti);
   d.strips = (d.geo.ny + MJ_LANES - 1) / MJ_LANES;
   d.items = d.strips * ((d.geo.nx + b->lu_seg - 1) / b->lu_seg);
+}
+
+// Limiter rows per wave for the cases resident at a commit: k_lu_batch runs
+// two 4-wave blocks per CU (one when the persistent form has one per CU), a
+// wave of `rows` rows marches rows + 6 fronts, and all cases' items share
+// the device, so the rows with the fewest rounds x fronts over the whole
+// table win (as rows_for_rounds does for one case).  A few large cases then
+// get short segments that fill the device in one round; many cases longer
+// ones with less warm-up.
+static int batch_rows(const wamp_batch* b) {
+  static const int cand[] = {4,   5,   6,   7,   8,   10,  12,   14,   16,   19,   22,
+                             26,  30,  34,  40,  48,  56,  64,   76,   90,   106,  128,
+                             152, 180, 216, 256, 300, 360, 430,  512,  600,  720,  860,
+                             1024, 1200, 1450, 1700, 2048, 2400, 2900, 3400, 4096};
+  const long long slots = 4LL * (b->lu_bpcu == 1 ? 1 : 2) * std::max(1, b->cu_count);
+  int max_nx = 4;
+  for (int k = 0; k < b->max_cases; ++k)
+    if (b->used[k]) max_nx = std::max(max_nx, b->desc[k].geo.nx);
+  int best = 26;
+  long long best_cost = -1;
+  for (int r : cand) {
+    if (r > max_nx && best_cost >= 0) break;
+    long long waves = 0;
+    for (int k = 0; k < b->max_cases; ++k) {
+      if (!b->used[k]) continue;
+      const Geo& g = b->desc[k].geo;
+      waves += (long long)((g.ny + MJ_LANES - 1) / MJ_LANES) * ((g.nx + r - 1) / r);
+    }
+    const long long cost = ((waves + slots - 1) / slots) * (r + 6);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = r;
+    }
+  }
+  return best;
 }
 
 static void graph_drop(wamp_batch* b) {
@@ -678,6 +714,7 @@ int wamp_batch_create(const wamp_scheme* scheme, int32_t precision, int32_t max_
   b->faces_bpcu = 3;
   b->lu_bpcu = 0;
   b->lu_seg = 16;
+  b->lu_seg_req = 16;
   b->desc.assign(max_cases, CaseDesc());
   b->used.assign(max_cases, 0);
   b->scratch.assign(max_cases, nullptr);
@@ -791,11 +828,13 @@ int wamp_batch_set_tuning(wamp_batch* b, int32_t tile, int32_t faces_bpcu, int32
   if (tile < 0 || tile > 1) return fail(-3, "tile must be 0 (8x32) or 1 (16x16)");
   if (faces_bpcu < 1 || faces_bpcu > 16) return fail(-3, "face blocks per CU must be in [1, 16]");
   if (lu_bpcu < 0 || lu_bpcu > 16) return fail(-3, "limiter blocks per CU must be in [0, 16]");
-  if (lu_seg < 4 || lu_seg > 4096) return fail(-3, "lu_seg must be in [4, 4096]");
+  if (lu_seg != 0 && (lu_seg < 4 || lu_seg > 4096))
+    return fail(-3, "lu_seg must be 0 (chosen at each commit) or in [4, 4096]");
   b->tile = tile;
   b->faces_bpcu = faces_bpcu;
   b->lu_bpcu = lu_bpcu;
-  b->lu_seg = lu_seg;
+  b->lu_seg_req = lu_seg;
+  if (lu_seg) b->lu_seg = lu_seg;
   b->committed = 0;
   return 0;
 }
@@ -807,6 +846,7 @@ int wamp_batch_commit(wamp_batch* b, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   graph_drop(b);
   const int n = b->max_cases;
+  if (b->lu_seg_req == 0) b->lu_seg = batch_rows(b);
   std::vector<int> off(2 * (n + 1), 0);
   for (int k = 0; k < n; ++k) {
     int tiles = 0, items = 0;
@@ -945,7 +985,7 @@ int wamp_batch_status(wamp_batch* b, double* t, double* dt, int32_t* done, void*
 
 int64_t wamp_batch_work(const wamp_batch* b, int32_t which) {
   if (!b) return -1;
-  return which == 0 ? b->total_tiles : b->total_items;
+  return which == 0 ? b->total_tiles : which == 1 ? b->total_items : b->lu_seg;
 }
 
 // ---- three-dimensional extension -------------------------------------------

@@ -29,13 +29,13 @@ class Tuning:
     tile: int = 0          # 0: 8x32 face tiles, 1: 16x16
     faces_bpcu: int = 3    # persistent face blocks per CU
     lu_bpcu: int = 0       # persistent limiter blocks (4 waves) per CU; 0: one wave per item
-    lu_seg: int = 16       # limiter rows per wave
+    lu_seg: int = 16       # limiter rows per wave; 0: chosen at each commit for the resident cases
 
     def as_tuple(self):
         return (self.tile, self.faces_bpcu, self.lu_bpcu, self.lu_seg)
 
 
-TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (0, 2) for s in (16, 26, 48)]
+TUNING_SPACE = [Tuning(t, f, l, s) for t in (0, 1) for f in (2, 3) for l in (0, 2) for s in (0, 26, 48)]
 
 
 class Batch:
@@ -154,7 +154,8 @@ class Batch:
         return np.array(t), np.array(dt), np.array(done, dtype=bool)
 
     def work(self):
-        return int(self.L.wamp_batch_work(self.h, 0)), int(self.L.wamp_batch_work(self.h, 1))
+        """(face tiles, limiter wave items, limiter rows per wave) of the committed table."""
+        return tuple(int(self.L.wamp_batch_work(self.h, w)) for w in range(3))
 
     # -- tuning ----------------------------------------------------------------
     def autotune(self, steps: int = 3, space=None, log=None) -> Tuning:

@@ -293,7 +293,7 @@ def serve_batched(reqs, scheme=SCHEME_MP5, precision=PREC_MIXED, max_slots: int 
                   log=None, track_busy: bool = False):
     """Run `reqs` (arrivals as offsets from the start) through a Server."""
     srv = Server(scheme=scheme, precision=precision, max_slots=max_slots, hbm_budget=hbm_budget,
-                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 26),
+                 round_steps=round_steps, tuning=tuning or Tuning(1, 3, 2, 0),
                  track_busy=track_busy)
     base = srv.clock()
     for r in reqs:
