@@ -845,6 +845,7 @@ int wamp_batch_commit(wamp_batch* b, void* stream) {
   if (!b) return fail(-1, "NULL batch");
   hipStream_t st = (hipStream_t)stream;
   graph_drop(b);
+  b->committed = 0;  // until the new table is on the device
   const int n = b->max_cases;
   if (b->lu_seg_req == 0) b->lu_seg = batch_rows(b);
   std::vector<int> off(2 * (n + 1), 0);
@@ -1385,8 +1386,10 @@ int wamp3_batch_commit(wamp3_batch* b, void* stream) {
   if (!b) return fail(-1, "NULL batch");
   hipStream_t st = (hipStream_t)stream;
   graph3_drop(b);
+  b->committed = 0;  // until the new table and its block counts are on the device
   const int n = b->max_cases;
   std::vector<int> off(B3_KINDS * (n + 1), 0);
+  int blocks[B3_KINDS];
   for (int kind = 0; kind < B3_KINDS; ++kind) {
     int* o = off.data() + kind * (n + 1);
     long long acc = 0;
@@ -1395,13 +1398,14 @@ int wamp3_batch_commit(wamp3_batch* b, void* stream) {
       if (acc > (1LL << 31) - 1) return fail(-3, "batch too large (more than 2^31 blocks)");
       o[k + 1] = (int)acc;
     }
-    b->blocks[kind] = o[n];
+    blocks[kind] = o[n];
   }
   if (int rc = hip_check(hipMemcpyAsync(b->desc_dev, b->desc.data(), sizeof(Case3Desc) * n,
                                         hipMemcpyHostToDevice, st), "hipMemcpyAsync(desc)")) return rc;
   if (int rc = hip_check(hipMemcpyAsync(b->off_dev, off.data(), sizeof(int) * off.size(),
                                         hipMemcpyHostToDevice, st), "hipMemcpyAsync(offsets)")) return rc;
   if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+  for (int kind = 0; kind < B3_KINDS; ++kind) b->blocks[kind] = blocks[kind];
   b->committed = 1;
   return 0;
 }
