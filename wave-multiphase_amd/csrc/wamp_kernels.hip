@@ -1,7 +1,7 @@
 // wamp_kernels.hip -- fused HIP kernels of the SSP-RK3 step for gfx950 (FP64).
 //
 // Per RK stage the reference launches ~20 kernels over 47 arrays
-// (SC:2031-2037).  Here a stage is two launches:
+// (SC:2031-2037).  Here a stage is two launches, a face kernel and a limiter:
 //
 //   k_faces        (wamp_faces.h) one LDS tile of 8x32 face positions: loads the 6
 //                  conserved fields of the tile plus a 3/4-cell halo once
@@ -9,21 +9,32 @@
 //                  mapping while loading), forms pressure, entropy, the
 //                  entropy indicator and the Ducros sensor once per cell,
 //                  then every thread evaluates one x-face and one y-face,
-//                  high order (fx_kernel SC:966-1133 / M10:868-1045) and
-//                  first order (fx_low_kernel SC:1303-1347), and stores the
-//                  28 face arrays.
-//   k_limit_update one 16x16 cell tile with a 3-cell halo frame in LDS: the
-//                  three Wong theta stages (pp_cell_theta / pp_blend,
-//                  SC:1397-1548) on shrinking frames, the final blended
-//                  fluxes, the SSP-RK3 combine with the alpha-source term
-//                  (rk3_kernel SC:1553-1611), the emergency repair
-//                  (SC:918-958), the alpha clamp (SC:851-854) and, on the
-//                  last stage, the CFL minimum (SC:1686-1705).
+//                  high order (fx_kernel SC:966-1133 / M10:868-1045) and,
+//                  unless the limiter forms them itself, first order
+//                  (fx_low_kernel SC:1303-1347), and stores the face arrays.
+//   limiter        the three Wong theta stages (pp_cell_theta / pp_blend,
+//                  SC:1397-1548), the final blended fluxes, the SSP-RK3
+//                  combine with the alpha-source term (rk3_kernel
+//                  SC:1553-1611), the emergency repair (SC:918-958), the alpha
+//                  clamp (SC:851-854) and, on the last stage, the CFL minimum
+//                  (SC:1686-1705), in one of three kernels with bitwise the
+//                  same results:
+//     k_limit_update  a 16x16 cell tile with a 3-cell halo frame in LDS
+//                     (grids up to 2^18 cells);
+//     k_lu_march      a wave per (58-column strip, segment of rows) marching
+//                     along i, neighbours by DPP lane shifts, carries in
+//                     registers and an LDS ring (larger grids, mixed storage);
+//     k_lu_lo         the march that also forms the first-order fluxes from
+//                     the stage input it streams (the FP64 step on larger
+//                     grids; the face kernel then writes only the 14
+//                     high-order planes).
+//   k_faces_batch / k_lu_batch run the same code over the case table of a
+//   batch.
 //
 // Compiled with -ffp-contract=off: every expression keeps the reference's
-// association, so the FP64 results agree with the reference's arithmetic
-// up to the ulp-level differences of pow/tanh between math libraries.  This
-// file instantiates the FP64 face kernels; the FP32-mixed ones live in
+// association.  pow (the entropy) and tanh (THINC) come from wamp_math.h, so
+// the FP64 results equal the oracle built with those functions bit for bit.
+// This file instantiates the FP64 face kernels; the FP32-mixed ones live in
 // wamp_faces_mixed.hip and are reached through wamp_launch_faces_mixed.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
