@@ -38,7 +38,6 @@
 // wamp_faces_mixed.hip and are reached through wamp_launch_faces_mixed.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include <type_traits>
 
 #include "wamp_device.h"
 #include "wamp_faces.h"
@@ -1369,11 +1368,12 @@ __global__ void __launch_bounds__(256) k_ic_triple(Geo G, Eos e, State U, Triple
 template <int BASE, int SHEAR, int FACEPP, typename T>
 static void launch_faces_t(const Geo& G, const Eos& e, const State& U, const Faces& F,
                            int vf_always, double t1, int low, hipStream_t st) {
-  // 16x16 tiles stage 2.07 cells per face position instead of 8x32's 2.29: the
-  // FP64 MUSCL kernels with the face positivity limiter (M10) gain from that
-  // (4096x2048: 1.008 instead of 1.025 ms per launch); the other schemes, and
-  // the mixed kernels, measured faster with 8x32.
-  using TL = std::conditional_t<(BASE == 0 && FACEPP == 1 && sizeof(T) == 8), Tile16x16, Tile8x32>;
+  // 16x16 tiles stage 2.07 cells per face position instead of 8x32's 2.29, and
+  // every FP64 scheme gains from that (4096x2048 per launch: M10 1.008 instead
+  // of 1.025 ms, SC 0.804 instead of 0.821, MP5 1.098 instead of 1.124).  The
+  // FP32-mixed kernels (wamp_faces_mixed.hip) measured faster with 8x32.
+  static_assert(sizeof(T) == 8, "the FP64 translation unit launches FP64 face kernels only");
+  using TL = Tile16x16;
   dim3 grid((G.ny + 1 + TL::TJ - 1) / TL::TJ, (G.nx + 1 + TL::TI - 1) / TL::TI);
   hipLaunchKernelGGL((k_faces<TL, BASE, SHEAR, FACEPP, T>), grid, dim3(FTHREADS), 0, st, G, e, U,
                      F, vf_always, t1, low);
