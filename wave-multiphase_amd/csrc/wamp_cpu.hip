@@ -100,7 +100,8 @@ void faces_t(CpuCase& c, const State& U, const Faces& F) {
     for (int t = lo; t < hi; ++t) {
       const int i0 = G.g - 1 + (t / tj) * CTile::TI;
       const int j0 = G.g - 1 + (t % tj) * CTile::TJ;
-      for (int r = 0; r < CTile::RN; ++r) tile_stage_cell<CTile>(G, c.eos, U, i0, j0, r, S);
+      using SS = StageSet<CTile, SHEAR>;
+      for (int k = 0; k < SS::N; ++k) tile_stage_cell<CTile>(G, c.eos, U, i0, j0, SS::cell(k), S);
       for (int k = 0; k < CTile::CXN; ++k) tile_sensor_x<CTile, SHEAR>(S, k);
       for (int k = 0; k < CTile::CYN; ++k) tile_sensor_y<CTile, SHEAR>(S, k);
       for (int p = 0; p < CTile::THREADS; ++p)

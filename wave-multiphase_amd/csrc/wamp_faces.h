@@ -273,6 +273,32 @@ WAMP_FN void tile_stage_cell(const Geo& G, const Eos& e, const State& U, int i0,
   S.q[5][r] = al;
   S.ent[r] = entropy(e, ar1, ar2, p, al);
 }
+// The rectangle cells phase A stages, by work index k < N: only those that
+// phases B and C read.  The x-faces and their indicators read LDS columns
+// 3..TJ+2 over all rows, the y-faces and theirs LDS rows 3..TI+2 over all
+// columns; the corner blocks outside that cross are never read, except, with
+// the Ducros sensor (SHEAR != 0), the 8 cells its velocity differences reach
+// at the sensor cells next to the corners: (2, 2), (2, TJ+3), (2, TJ+4),
+// (TI+3, 2), (TI+3, TJ+3), (TI+3, TJ+4), (TI+4, 2), (TI+4, TJ+3).  At 16x16
+// that is 480 (488) of the 529 cells: two passes of the block's 256 threads
+// instead of three.
+template <class TL, int SHEAR>
+struct StageSet {
+  static constexpr int BAND = TL::TI * TL::RJ;        // LDS rows 3..TI+2, every column
+  static constexpr int CROSS = BAND + 7 * TL::TJ;     // + rows 0-2, TI+3..TI+6 at columns 3..TJ+2
+  static constexpr int N = CROSS + (SHEAR != 0 ? 8 : 0);
+  WAMP_FN static int cell(int k) {
+    if (k < BAND) return 3 * TL::RJ + k;
+    if (k < CROSS) {
+      const int q = k - BAND, rr = q / TL::TJ;  // rr 0..6: rows 0-2, TI+3..TI+6
+      return (rr < 3 ? rr : TL::TI + rr) * TL::RJ + 3 + q % TL::TJ;
+    }
+    const int q = k - CROSS;  // 0..7, the Ducros cells in the order listed above
+    const int row = q < 3 ? 2 : (q < 6 ? TL::TI + 3 : TL::TI + 4);
+    const int col = q % 3 == 0 ? 2 : TL::TJ + 2 + q % 3;
+    return row * TL::RJ + col;
+  }
+};
 template <class TL, int SHEAR>
 WAMP_FN void tile_sensor_x(FaceSmem<TL>& S, int k) {
   constexpr int FTJ = TL::TJ, FRJ = TL::RJ;
@@ -337,7 +363,8 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   const int tid = threadIdx.x;
   const int i0 = G.g - 1 + bi * TL::TI;
   const int j0 = G.g - 1 + bj * TL::TJ;
-  for (int r = tid; r < TL::RN; r += FTHREADS) tile_stage_cell<TL>(G, e, U, i0, j0, r, S);
+  using SS = StageSet<TL, SHEAR>;
+  for (int k = tid; k < SS::N; k += FTHREADS) tile_stage_cell<TL>(G, e, U, i0, j0, SS::cell(k), S);
   __syncthreads();
   for (int k = tid; k < TL::CXN; k += FTHREADS) tile_sensor_x<TL, SHEAR>(S, k);
   for (int k = tid; k < TL::CYN; k += FTHREADS) tile_sensor_y<TL, SHEAR>(S, k);
