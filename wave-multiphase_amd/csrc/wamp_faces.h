@@ -366,8 +366,17 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   using SS = StageSet<TL, SHEAR>;
   for (int k = tid; k < SS::N; k += FTHREADS) tile_stage_cell<TL>(G, e, U, i0, j0, SS::cell(k), S);
   __syncthreads();
-  for (int k = tid; k < TL::CXN; k += FTHREADS) tile_sensor_x<TL, SHEAR>(S, k);
-  for (int k = tid; k < TL::CYN; k += FTHREADS) tile_sensor_y<TL, SHEAR>(S, k);
+  // Phase B: one x and one y indicator per thread, then the remaining ones
+  // in a third pass whose x and y parts start on separate waves, so that no
+  // wave runs more than three of them (in order, wave 0 ran four).
+  constexpr int LX = TL::CXN - FTHREADS, LY = TL::CYN - FTHREADS, OY = (LX + 63) / 64 * 64;
+  static_assert(LX > 0 && LY > 0 && OY + LY <= FTHREADS, "tile shape outside phase B's three passes");
+  tile_sensor_x<TL, SHEAR>(S, tid);
+  tile_sensor_y<TL, SHEAR>(S, tid);
+  if (tid < LX)
+    tile_sensor_x<TL, SHEAR>(S, FTHREADS + tid);
+  else if (tid >= OY && tid < OY + LY)
+    tile_sensor_y<TL, SHEAR>(S, FTHREADS + tid - OY);
   __syncthreads();
   tile_faces_at<TL, BASE, SHEAR, FACEPP, T>(G, e, F, vf_always, t1, low, i0, j0, tid, S);
 }
