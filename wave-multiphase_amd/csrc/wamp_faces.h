@@ -250,19 +250,29 @@ WAMP_FN void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) 
 //   B  entropy indicator (cator2_val SC:318-325) and Ducros sensor once per
 //      cell along each sweep direction (k < TL::CXN, k < TL::CYN);
 //   C  the x-face and the y-face at tile position t (t < TL::THREADS).
+// Phase A in two halves, so that a thread can issue the loads of all its
+// cells before it computes any of them: the conserved values of rectangle
+// cell r (its boundary-mapped cell) and the slip signs ...
+struct StagedCell {
+  double v[6];
+  double sru, srv;
+};
 template <class TL>
-WAMP_FN void tile_stage_cell(const Geo& G, const Eos& e, const State& U, int i0, int j0, int r,
-                             FaceSmem<TL>& S) {
+WAMP_FN void tile_load_cell(const Geo& G, const State& U, int i0, int j0, int r, StagedCell& c) {
   constexpr int FRJ = TL::RJ;
   const int i = i0 - 3 + r / FRJ;
   const int j = j0 - 3 + r % FRJ;
-  double sru, srv;
-  const size_t c = map_cell(G, i, j, sru, srv);
-  double ar1 = U.f[0][c], ar2 = U.f[1][c];
-  double ru = U.f[2][c], rv = U.f[3][c];
-  double E = U.f[4][c], al = U.f[5][c];
-  if (sru < 0.0) ru = -1.0 * ru;
-  if (srv < 0.0) rv = -1.0 * rv;
+  const size_t o = map_cell(G, i, j, c.sru, c.srv);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) c.v[m] = U.f[m][o];
+}
+// ... then the slip signs applied, pressure and entropy, and the LDS stores.
+template <class TL>
+WAMP_FN void tile_store_cell(const Eos& e, const StagedCell& c, int r, FaceSmem<TL>& S) {
+  const double ar1 = c.v[0], ar2 = c.v[1], E = c.v[4], al = c.v[5];
+  double ru = c.v[2], rv = c.v[3];
+  if (c.sru < 0.0) ru = -1.0 * ru;
+  if (c.srv < 0.0) rv = -1.0 * rv;
   double uu, vv;
   double p = pressure(e, ar1, ar2, ru, rv, E, al, uu, vv);
   S.q[0][r] = ar1;
@@ -272,6 +282,13 @@ WAMP_FN void tile_stage_cell(const Geo& G, const Eos& e, const State& U, int i0,
   S.q[4][r] = p;
   S.q[5][r] = al;
   S.ent[r] = entropy(e, ar1, ar2, p, al);
+}
+template <class TL>
+WAMP_FN void tile_stage_cell(const Geo& G, const Eos& e, const State& U, int i0, int j0, int r,
+                             FaceSmem<TL>& S) {
+  StagedCell c;
+  tile_load_cell<TL>(G, U, i0, j0, r, c);
+  tile_store_cell<TL>(e, c, r, S);
 }
 // The rectangle cells phase A stages, by work index k < N: only those that
 // phases B and C read.  The x-faces and their indicators read LDS columns
@@ -363,8 +380,22 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   const int tid = threadIdx.x;
   const int i0 = G.g - 1 + bi * TL::TI;
   const int j0 = G.g - 1 + bj * TL::TJ;
+  // Phase A: every load of the thread's cells first (a thread without a cell
+  // in the last pass loads its first cell again), then their arithmetic, so
+  // the wave waits on memory once instead of once per pass.
   using SS = StageSet<TL, SHEAR>;
-  for (int k = tid; k < SS::N; k += FTHREADS) tile_stage_cell<TL>(G, e, U, i0, j0, SS::cell(k), S);
+  constexpr int NP = (SS::N + FTHREADS - 1) / FTHREADS;
+  StagedCell cells[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int k = tid + q * FTHREADS;
+    tile_load_cell<TL>(G, U, i0, j0, SS::cell(k < SS::N ? k : tid), cells[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const int k = tid + q * FTHREADS;
+    if (k < SS::N) tile_store_cell<TL>(e, cells[q], SS::cell(k), S);
+  }
   __syncthreads();
   // Phase B: one x and one y indicator per thread, then the remaining ones
   // in a third pass whose x and y parts start on separate waves, so that no
