@@ -385,6 +385,7 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   // the wave waits on memory once instead of once per pass.
   using SS = StageSet<TL, SHEAR>;
   constexpr int NP = (SS::N + FTHREADS - 1) / FTHREADS;
+  static_assert(SS::N >= FTHREADS, "every thread stages a first cell");
   StagedCell cells[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
