@@ -246,7 +246,7 @@ WAMP_FN void face_low(const Eos& e, const FaceSmem<TL>& S, int r0, double* out) 
 // function of one work index so that the device runs a phase with one
 // thread per index (a barrier between phases) and the host with a loop:
 //   A  stage cell r of the rectangle: conserved values, pressure and
-//      entropy once per cell (r < TL::RN);
+//      entropy once per cell (the cells StageSet lists);
 //   B  entropy indicator (cator2_val SC:318-325) and Ducros sensor once per
 //      cell along each sweep direction (k < TL::CXN, k < TL::CYN);
 //   C  the x-face and the y-face at tile position t (t < TL::THREADS).
@@ -400,7 +400,8 @@ __device__ __forceinline__ void faces_tile(const Geo& G, const Eos& e, const Sta
   __syncthreads();
   // Phase B: one x and one y indicator per thread, then the remaining ones
   // in a third pass whose x and y parts start on separate waves, so that no
-  // wave runs more than three of them (in order, wave 0 ran four).
+  // wave runs more than three of them (a strided loop over each kind in turn
+  // would give wave 0 four).
   constexpr int LX = TL::CXN - FTHREADS, LY = TL::CYN - FTHREADS, OY = (LX + 63) / 64 * 64;
   static_assert(LX > 0 && LY > 0 && OY + LY <= FTHREADS, "tile shape outside phase B's three passes");
   tile_sensor_x<TL, SHEAR>(S, tid);
